@@ -1,0 +1,145 @@
+#!/usr/bin/env python3
+"""Throughput of the GPU geodesic tracer on the Kerr a=0.998 1024^2 4-spp frame.
+
+One step = one full frame: 1024 x 1024 pixels x 4 samples (seeded jitter,
+seed 1234), thin Novikov–Thorne disk from the ISCO to 20 M, 5000-star
+background, observer r=50 M, theta=80 deg, 90 deg field of view, tolerance
+1e-8, at most 10000 steps, FP64 state.  The output frame stays in device
+memory (the rate including the host copy is reported separately).
+
+Prints one JSON line.  Runs on one GPU; --gpus above 1 exits with a message.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+W = H = 1024
+SPP = 4
+SEED = 1234
+FP64_PEAK_TFLOPS = 78.6           # MI355X FP64 vector peak
+# Algorithmic FP64 work of one Dormand–Prince attempt in the reference
+# formulation: 7 Kerr right-hand sides of ~170 flops plus ~450 flops of stage
+# combinations, error norm and control.
+ALGO_FLOPS_PER_ATTEMPT = 1640.0
+
+
+def c2_params(grrt):
+    return grrt.make_params(width=W, height=H, metric=grrt.METRIC_KERR, spin=0.998,
+                            observer_r=50.0, observer_theta=math.radians(80.0),
+                            observer_phi=0.0, fov=math.radians(90.0), disk=True,
+                            disk_outer=20.0, stars=True, tolerance=1e-8, max_steps=10000,
+                            spp=SPP)
+
+
+def cpu_baseline(target_s: float = 15.0) -> dict:
+    """The CPU oracle built with the reference's release flags, OpenMP over
+    the host cores, on a random sample of the same frame's rays."""
+    import numpy as np
+    import oracle as O
+
+    threads = max(1, min(16, os.cpu_count() or 1))
+    sc = O.scene(width=W, height=H, spin_a=0.998, r_obs=50.0, theta_obs=math.radians(80.0),
+                 fov=math.radians(90.0), disk=True, r_out=20.0, stars=True)
+    rng = np.random.default_rng(2024)
+    n_total = W * H * SPP
+    probe = np.sort(rng.choice(n_total, 20000, replace=False))
+    t0 = time.perf_counter()
+    O.trace_samples(sc, O.JITTER_SEEDED, SPP, SEED, probe, threads=threads, variant="fast")
+    rate = len(probe) / (time.perf_counter() - t0)
+    n = int(min(n_total, max(20000, rate * target_s)))
+    ids = np.sort(rng.choice(n_total, n, replace=False))
+    t0 = time.perf_counter()
+    O.trace_samples(sc, O.JITTER_SEEDED, SPP, SEED, ids, threads=threads, variant="fast")
+    dt = time.perf_counter() - t0
+    return {"value": round(n / dt / 1e6, 5), "unit": "Mrays/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{n} rays drawn uniformly from the same 1024^2x4 frame, {dt:.1f} s, "
+                      f"oracle built -O3 -ffast-math -mavx2 -mfma, OpenMP guided"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--tune", default="", help="block,blocks_per_cu,refill_min")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(rays are independent; the multi-GPU split is described in DESIGN.md)",
+              file=sys.stderr)
+        return 2
+
+    import grrt_amd as grrt
+
+    params = c2_params(grrt)
+    ctx = grrt.Context(params)
+    ctx.set_sampling(grrt.SAMPLING_SEEDED, SPP, SEED)
+    if args.tune:
+        b, bpc, rf = (int(x) for x in args.tune.split(","))
+        ctx.set_tuning(b, bpc, rf)
+    for _ in range(args.warmup):
+        ctx.render_device()
+    trace_ms, attempts, steps = [], 0, 0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.render_device()          # returns after the frame is complete
+        s = ctx.stats()
+        trace_ms.append(s.trace_ms)
+        attempts += s.attempts
+        steps += s.steps
+    wall = time.perf_counter() - t0
+    ms_per_step = wall / args.steps * 1e3
+    rays = W * H * SPP
+    value = rays / (ms_per_step / 1e3) / 1e6
+    avg_trace_s = sum(trace_ms) / len(trace_ms) / 1e3
+    att_per_launch = attempts / args.steps
+    achieved = ALGO_FLOPS_PER_ATTEMPT * att_per_launch / avg_trace_s / 1e12
+    # host-copy-inclusive rate (grrt_render into a host framebuffer)
+    t1 = time.perf_counter()
+    ctx.render()
+    host_ms = (time.perf_counter() - t1) * 1e3
+
+    out = {
+        "metric": "Mrays/s Kerr a=0.998 1024^2 4spp (plain FP64 form)",
+        "value": round(value, 3),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic: camera rays of the Kerr a=0.998 thin-disk scene",
+        "config": {"workload": "Kerr a=0.998 M=1, 1024x1024, 4 spp seeded jitter (seed 1234), "
+                               "r_obs=50 theta=80deg fov=90deg, thin disk r_isco..20M, stars, "
+                               "tol 1e-8, max 10000 steps",
+                   "rays_per_step": rays, "parallelism": "single GPU"},
+        "geodesic_steps_per_s": round(steps / args.steps / (ms_per_step / 1e3), 1),
+        "attempts_per_launch": int(att_per_launch),
+        "trace_kernel_ms": round(avg_trace_s * 1e3, 3),
+        "host_copy_inclusive_ms": round(host_ms, 3),
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3),
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    ctx.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,281 @@
+"""Python binding of libgrrt.so, the MI355X GRRT backend.
+
+The binding mirrors the reference's C ABI (include/grrt/api.h) one to one
+through ctypes, the way the reference's planned Blender binding would have
+(gr-raytracer-blender-spec_1.md:566-600), plus the ``grrt_ext_*`` entry
+points for sampling control, per-ray records and counters.  There is no
+CPU fallback: if the library is not built, or no GPU is present,
+``Context(...)`` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libgrrt.so")
+
+METRIC_SCHWARZSCHILD, METRIC_KERR = 0, 1
+BACKEND_CPU, BACKEND_GPU = 0, 1
+BG_BLACK, BG_STARS, BG_TEXTURE = 0, 1, 2
+SAMPLING_HASH, SAMPLING_CENTER, SAMPLING_SEEDED = 0, 1, 2
+CLASS_CAPTURED, CLASS_ESCAPED, CLASS_DISK, CLASS_MAXSTEPS = 0, 1, 2, 3
+
+
+class GRRTParams(C.Structure):
+    """Field-for-field copy of GRRTParams (include/grrt/types.h)."""
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int),
+        ("metric_type", C.c_int), ("mass", C.c_double), ("spin", C.c_double),
+        ("observer_r", C.c_double), ("observer_theta", C.c_double),
+        ("observer_phi", C.c_double), ("fov", C.c_double),
+        ("cam_yaw", C.c_double), ("cam_pitch", C.c_double), ("cam_roll", C.c_double),
+        ("disk_enabled", C.c_int), ("disk_inner", C.c_double), ("disk_outer", C.c_double),
+        ("disk_temperature", C.c_double),
+        ("disk_volumetric", C.c_int), ("disk_alpha", C.c_double),
+        ("disk_turbulence", C.c_double), ("disk_seed", C.c_int),
+        ("disk_noise_scale", C.c_double), ("disk_noise_octaves", C.c_int),
+        ("mass_solar", C.c_double), ("eddington_fraction", C.c_double),
+        ("background_type", C.c_int), ("background_texture_path", C.c_char_p),
+        ("integrator_tolerance", C.c_double), ("integrator_max_steps", C.c_int),
+        ("samples_per_pixel", C.c_int), ("thread_count", C.c_int), ("backend", C.c_int),
+        ("num_frequency_bins", C.c_int), ("frequency_bins_hz", C.POINTER(C.c_double)),
+    ]
+
+
+assert C.sizeof(GRRTParams) == 240
+
+
+class GRRTRayRecord(C.Structure):
+    _fields_ = [
+        ("color", C.c_double * 3), ("x_final", C.c_double * 4), ("p_final", C.c_double * 4),
+        ("r_hit", C.c_double), ("g_hit", C.c_double), ("h_max", C.c_double),
+        ("h_final", C.c_double),
+        ("term", C.c_int32), ("cls", C.c_int32), ("steps", C.c_int32),
+        ("attempts", C.c_int32), ("n_cross", C.c_int32), ("pad", C.c_int32),
+    ]
+
+
+RECORD_DTYPE = np.dtype([
+    ("color", "<f8", 3), ("x_final", "<f8", 4), ("p_final", "<f8", 4),
+    ("r_hit", "<f8"), ("g_hit", "<f8"), ("h_max", "<f8"), ("h_final", "<f8"),
+    ("term", "<i4"), ("cls", "<i4"), ("steps", "<i4"), ("attempts", "<i4"),
+    ("n_cross", "<i4"), ("pad", "<i4"),
+])
+assert RECORD_DTYPE.itemsize == C.sizeof(GRRTRayRecord) == 144
+
+
+class GRRTStats(C.Structure):
+    _fields_ = [
+        ("trace_ms", C.c_double), ("finalize_ms", C.c_double), ("total_ms", C.c_double),
+        ("rays", C.c_int64), ("steps", C.c_int64), ("attempts", C.c_int64),
+        ("deriv_evals", C.c_int64), ("algo_flops", C.c_double),
+        ("cancelled", C.c_int32), ("pad", C.c_int32),
+    ]
+
+
+PROGRESS_FN = C.CFUNCTYPE(None, C.c_float, C.c_void_p)
+
+# Every symbol the C headers declare: (name, restype, argtypes).
+_P = C.c_void_p
+API = [
+    ("grrt_create", _P, [C.POINTER(GRRTParams)]),
+    ("grrt_destroy", None, [_P]),
+    ("grrt_update_params", None, [_P, C.POINTER(GRRTParams)]),
+    ("grrt_render", C.c_int, [_P, C.POINTER(C.c_float)]),
+    ("grrt_render_cb", C.c_int, [_P, C.POINTER(C.c_float), PROGRESS_FN, _P]),
+    ("grrt_render_tile", C.c_int, [_P, C.POINTER(C.c_float), C.c_int, C.c_int, C.c_int, C.c_int]),
+    ("grrt_cancel", None, [_P]),
+    ("grrt_progress", C.c_float, [_P]),
+    ("grrt_error", C.c_char_p, [_P]),
+    ("grrt_cuda_available", C.c_int, []),
+    ("grrt_last_error", C.c_char_p, []),
+    ("grrt_tonemap", None, [C.POINTER(C.c_float), C.c_int, C.c_int]),
+    ("grrt_debug_pixel", None, [_P, C.c_int, C.c_int]),
+    ("grrt_set_frequency_bins", None, [_P, C.POINTER(C.c_double), C.c_int]),
+    ("grrt_render_spectral", C.c_int, [_P, C.POINTER(C.c_double), C.c_int, C.c_int]),
+    ("grrt_render_spectral_cb", C.c_int,
+     [_P, C.POINTER(C.c_double), C.c_int, C.c_int, PROGRESS_FN, _P]),
+    ("grrt_render_spectral_to_fits_cb", C.c_int,
+     [_P, C.c_char_p, C.c_int, C.c_int, PROGRESS_FN, _P]),
+    ("grrt_ext_set_sampling", C.c_int, [_P, C.c_int, C.c_int, C.c_uint64]),
+    ("grrt_ext_set_records", C.c_int, [_P, C.c_int]),
+    ("grrt_ext_num_samples", C.c_int64, [_P]),
+    ("grrt_ext_get_records", C.c_int, [_P, _P, C.c_int64]),
+    ("grrt_ext_get_classes", C.c_int, [_P, _P, C.c_int64]),
+    ("grrt_ext_last_stats", C.c_int, [_P, C.POINTER(GRRTStats)]),
+    ("grrt_ext_render_device", C.c_int, [_P, _P]),
+    ("grrt_ext_set_tuning", C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
+]
+
+_lib: C.CDLL | None = None
+
+
+def build(jobs: int = 8) -> None:
+    """Compile libgrrt.so for gfx950 with the committed Makefile."""
+    subprocess.run(["make", "-s", f"-j{jobs}", "-C", os.path.join(_HERE, "csrc")], check=True)
+
+
+def lib() -> C.CDLL:
+    """Load libgrrt.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run grrt_amd.build()")
+        L = C.CDLL(LIB_PATH)
+        for name, res, args in API:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def gpu_available() -> bool:
+    return bool(lib().grrt_cuda_available())
+
+
+def make_params(width=256, height=256, metric=METRIC_KERR, spin=0.998, mass=1.0,
+                observer_r=50.0, observer_theta=1.3962634, observer_phi=0.0,
+                fov=1.5707963, disk=True, disk_inner=0.0, disk_outer=20.0,
+                disk_temperature=0.0, stars=False, tolerance=1e-8, max_steps=10000,
+                spp=1, volumetric=False) -> GRRTParams:
+    p = GRRTParams()
+    p.width, p.height = width, height
+    p.metric_type = metric
+    p.mass, p.spin = mass, spin
+    p.observer_r, p.observer_theta, p.observer_phi, p.fov = observer_r, observer_theta, observer_phi, fov
+    p.disk_enabled = 1 if disk else 0
+    p.disk_inner, p.disk_outer, p.disk_temperature = disk_inner, disk_outer, disk_temperature
+    p.disk_volumetric = 1 if volumetric else 0
+    p.background_type = BG_STARS if stars else BG_BLACK
+    p.integrator_tolerance, p.integrator_max_steps = tolerance, max_steps
+    p.samples_per_pixel = spp
+    p.backend = BACKEND_GPU
+    return p
+
+
+class Context:
+    """Owns one GRRTContext (grrt_create ... grrt_destroy)."""
+
+    def __init__(self, params: GRRTParams):
+        self._L = lib()
+        self.params = params
+        self._h = self._L.grrt_create(C.byref(params))
+        if not self._h:
+            err = self._L.grrt_last_error()
+            raise RuntimeError("grrt_create failed: " + (err.decode() if err else "unknown"))
+
+    def close(self):
+        if self._h:
+            self._L.grrt_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc: int, what: str) -> int:
+        if rc < 0:
+            err = self._L.grrt_error(self._h)
+            raise RuntimeError(f"{what} failed: " + (err.decode() if err else "unknown"))
+        return rc
+
+    @property
+    def error(self):
+        e = self._L.grrt_error(self._h)
+        return e.decode() if e else None
+
+    def set_sampling(self, mode: int, spp: int = 1, seed: int = 0):
+        self._check(self._L.grrt_ext_set_sampling(self._h, mode, spp, seed), "set_sampling")
+
+    def set_records(self, enable: bool = True):
+        self._check(self._L.grrt_ext_set_records(self._h, 1 if enable else 0), "set_records")
+
+    def set_tuning(self, block=0, blocks_per_cu=0, refill_min=0):
+        self._check(self._L.grrt_ext_set_tuning(self._h, block, blocks_per_cu, refill_min), "set_tuning")
+
+    def render(self, progress=None) -> np.ndarray:
+        fb = np.zeros((self.params.height, self.params.width, 4), np.float32)
+        ptr = fb.ctypes.data_as(C.POINTER(C.c_float))
+        if progress is None:
+            rc = self._L.grrt_render(self._h, ptr)
+        else:
+            cb = PROGRESS_FN(lambda f, _ud: progress(f))
+            rc = self._L.grrt_render_cb(self._h, ptr, cb, None)
+        self._check(rc, "grrt_render")
+        self.cancelled = rc == 1
+        return fb
+
+    def render_tile(self, x, y, w, h) -> np.ndarray:
+        buf = np.zeros((h, w, 4), np.float32)
+        self._check(self._L.grrt_render_tile(self._h, buf.ctypes.data_as(C.POINTER(C.c_float)),
+                                             x, y, w, h), "grrt_render_tile")
+        return buf
+
+    def render_device(self, d_ptr: int | None = None) -> int:
+        return self._check(self._L.grrt_ext_render_device(self._h, d_ptr), "grrt_ext_render_device")
+
+    def num_samples(self) -> int:
+        return int(self._L.grrt_ext_num_samples(self._h))
+
+    def classes(self, n: int | None = None) -> np.ndarray:
+        n = self.num_samples() if n is None else n
+        out = np.zeros(n, np.uint8)
+        self._check(self._L.grrt_ext_get_classes(self._h, out.ctypes.data, n), "get_classes")
+        return out
+
+    def records(self, n: int | None = None) -> np.ndarray:
+        n = self.num_samples() if n is None else n
+        out = np.zeros(n, RECORD_DTYPE)
+        self._check(self._L.grrt_ext_get_records(self._h, out.ctypes.data, n), "get_records")
+        return out
+
+    def stats(self) -> GRRTStats:
+        s = GRRTStats()
+        self._check(self._L.grrt_ext_last_stats(self._h, C.byref(s)), "last_stats")
+        return s
+
+    def cancel(self):
+        self._L.grrt_cancel(self._h)
+
+    def progress(self) -> float:
+        return float(self._L.grrt_progress(self._h))
+
+    def update_params(self, params: GRRTParams):
+        self.params = params
+        self._L.grrt_update_params(self._h, C.byref(params))
+        if self.error:
+            raise RuntimeError("grrt_update_params failed: " + self.error)
+
+    def set_frequency_bins(self, freqs):
+        arr = np.ascontiguousarray(freqs, np.float64)
+        self._freqs = arr
+        self._L.grrt_set_frequency_bins(self._h, arr.ctypes.data_as(C.POINTER(C.c_double)), len(arr))
+
+    def render_spectral(self, width=None, height=None) -> np.ndarray:
+        w = self.params.width if width is None else width
+        h = self.params.height if height is None else height
+        n = len(getattr(self, "_freqs", []))
+        buf = np.zeros((h, w, max(n, 1)), np.float64)
+        self._check(self._L.grrt_render_spectral(self._h, buf.ctypes.data_as(C.POINTER(C.c_double)),
+                                                 w, h), "grrt_render_spectral")
+        return buf
+
+
+def tonemap(fb: np.ndarray) -> np.ndarray:
+    out = np.ascontiguousarray(fb, np.float32).copy()
+    h, w = out.shape[:2]
+    lib().grrt_tonemap(out.ctypes.data_as(C.POINTER(C.c_float)), w, h)
+    return out

@@ -1,0 +1,517 @@
+// The GRRT C ABI (include/grrt/api.h) and its extensions (include/grrt_ext.h)
+// over the HIP tracer.  Every entry point catches exceptions at the boundary
+// and reports failures through grrt_error / grrt_last_error.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstddef>
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <string>
+#include <thread>
+
+#include "context.hpp"
+#include "grrt/api.h"
+#include "grrt_ext.h"
+
+static_assert(sizeof(GRRTParams) == 240, "GRRTParams must keep the reference's LP64 layout");
+static_assert(offsetof(GRRTParams, mass) == 16 && offsetof(GRRTParams, cam_yaw) == 64 &&
+                  offsetof(GRRTParams, disk_enabled) == 88 && offsetof(GRRTParams, disk_volumetric) == 120 &&
+                  offsetof(GRRTParams, disk_seed) == 144 && offsetof(GRRTParams, mass_solar) == 168 &&
+                  offsetof(GRRTParams, background_type) == 184 &&
+                  offsetof(GRRTParams, integrator_tolerance) == 200 &&
+                  offsetof(GRRTParams, samples_per_pixel) == 212 && offsetof(GRRTParams, backend) == 220 &&
+                  offsetof(GRRTParams, frequency_bins_hz) == 232,
+              "GRRTParams field offsets must match the reference");
+static_assert(sizeof(GRRTRayRecord) == 144, "GRRTRayRecord layout");
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct HipFailure : std::exception {
+    std::string msg;
+    explicit HipFailure(std::string m) : msg(std::move(m)) {}
+    const char* what() const noexcept override { return msg.c_str(); }
+};
+
+void check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw HipFailure(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+void upload(grrt_amd::DevBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
+    check(b.ensure(v.size()), "hipMalloc(table)");
+    check(hipMemcpyAsync(b.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st),
+          "hipMemcpyAsync(table)");
+}
+
+// (Re)build the scene tables from ctx->params.
+void build_scene(GRRTContext* ctx) {
+    using namespace grrt_amd;
+    ctx->res = resolve_params(ctx->params);
+    const Resolved& r = ctx->res;
+    if (r.W <= 0 || r.H <= 0) throw HipFailure("width and height must be positive");
+    ctx->cam = build_camera(r.M, r.a, r.r_obs, r.th_obs, r.ph_obs);
+    std::vector<double> flux, spec;
+    ctx->flux_max = 0.0;
+    if (r.disk) ctx->flux_max = build_flux_table(r.M, r.a, r.r_in, r.r_out, flux);
+    else flux.assign(kFluxN, 0.0);
+    build_chromaticity_table(spec);
+    upload(ctx->flux, flux, ctx->stream);
+    upload(ctx->spec, spec, ctx->stream);
+    if (r.stars && ctx->star_start.ptr == nullptr) {
+        StarTable t;
+        build_star_table(t);
+        upload(ctx->star_start, t.start, ctx->stream);
+        upload(ctx->star_tpb, t.tpb, ctx->stream);
+    }
+    ctx->sampling_spp = r.spp;
+    check(hipStreamSynchronize(ctx->stream), "upload tables");
+}
+
+int samples_per_pixel(int mode, int spp, int& grid) {
+    if (mode == GRRT_SAMPLING_CENTER) { grid = 1; return 1; }
+    if (mode == GRRT_SAMPLING_HASH) {
+        grid = std::max(1, static_cast<int>(std::sqrt(static_cast<double>(std::max(spp, 1)))));
+        return grid * grid;
+    }
+    grid = 0;
+    return std::max(spp, 1);
+}
+
+// Render the pixel window [x0,x0+tw) x [y0,y0+th) into device memory `out`
+// (tw*th*4 floats).  Returns 0, or 1 if cancelled.
+int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, float* d_out,
+                  grrt_progress_fn progress, void* ud) {
+    using namespace grrt_amd;
+    const auto t_start = std::chrono::steady_clock::now();
+    const Resolved& r = ctx->res;
+    if (r.volumetric)
+        throw HipFailure("the volumetric disk is not available in this build of the GPU backend");
+    int grid = 1;
+    const int nS = samples_per_pixel(ctx->sampling_mode, ctx->sampling_spp, grid);
+    const long long npix = static_cast<long long>(tw) * th;
+    const long long n_rays = npix * nS;
+    check(ctx->sample_rgb.ensure(static_cast<size_t>(3 * n_rays)), "hipMalloc(samples)");
+    check(ctx->sample_cls.ensure(static_cast<size_t>(n_rays)), "hipMalloc(classes)");
+    if (ctx->keep_records) check(ctx->records.ensure(static_cast<size_t>(n_rays)), "hipMalloc(records)");
+    check(ctx->counters.ensure(1), "hipMalloc(counters)");
+
+    TraceParams P{};
+    P.M = r.M;
+    P.a = r.a;
+    P.r_term = r.r_h + 0.01;
+    P.r_escape = 1000.0;
+    P.tol = r.tol;
+    P.max_steps = r.max_steps;
+    P.W = r.W;
+    P.H = r.H;
+    P.r_obs = r.r_obs;
+    P.th_obs = r.th_obs;
+    P.ph_obs = r.ph_obs;
+    P.fov = r.fov;
+    P.e0t = ctx->cam.e0[0];
+    for (int m = 0; m < 4; ++m) {
+        P.e1[m] = ctx->cam.e1[m];
+        P.e2[m] = ctx->cam.e2[m];
+        P.e3[m] = ctx->cam.e3[m];
+    }
+    P.g00 = ctx->cam.g[0][0];
+    P.g03 = ctx->cam.g[0][3];
+    P.g11 = ctx->cam.g[1][1];
+    P.g22 = ctx->cam.g[2][2];
+    P.g33 = ctx->cam.g[3][3];
+    P.mode = ctx->sampling_mode;
+    P.grid = grid;
+    P.nS = nS;
+    P.cell = grid > 0 ? 1.0 / grid : 1.0;
+    P.seed = ctx->sampling_seed;
+    P.x0 = x0;
+    P.y0 = y0;
+    P.tw = tw;
+    P.th = th;
+    P.n_rays = n_rays;
+    P.disk = r.disk;
+    P.r_in = r.r_in;
+    P.r_out = r.r_out;
+    P.t_peak = r.t_disk;
+    P.flux_max = ctx->flux_max;
+    P.ut_obs = 1.0 / std::sqrt(1.0 - 2.0 * r.M / r.r_obs);
+    P.flux_lut = ctx->flux.ptr;
+    P.spec_lut = ctx->spec.ptr;
+    P.stars = r.stars;
+    P.star_start = ctx->star_start.ptr;
+    P.star_tpb = ctx->star_tpb.ptr;
+    P.sample_rgb = ctx->sample_rgb.ptr;
+    P.sample_cls = ctx->sample_cls.ptr;
+    P.records = ctx->keep_records ? ctx->records.ptr : nullptr;
+    P.counters = ctx->counters.ptr;
+    P.cancel = ctx->cancel_flag_dev;
+    P.refill_min = std::clamp(ctx->tune_refill, 1, 64);
+
+    *ctx->cancel_flag = 0;
+    ctx->progress.store(0.0f);
+    const int threads = ctx->tune_block;
+    const int blocks = std::max(1, ctx->num_cus * ctx->tune_blocks_per_cu);
+    check(launch_reset(P.counters, ctx->stream), "reset kernel");
+    check(hipEventRecord(ctx->ev[0], ctx->stream), "hipEventRecord");
+    check(launch_trace_thin(P, blocks, threads, ctx->stream), "trace kernel launch");
+    check(hipEventRecord(ctx->ev[1], ctx->stream), "hipEventRecord");
+    check(launch_finalize(P.sample_rgb, d_out, npix, nS, P.counters, ctx->stream), "finalize launch");
+    check(hipEventRecord(ctx->ev[2], ctx->stream), "hipEventRecord");
+
+    // Wait on the caller's thread, reporting coarse progress from elapsed
+    // work; the exact count is read once the kernels end.
+    if (progress) {
+        while (hipEventQuery(ctx->ev[2]) == hipErrorNotReady)
+            std::this_thread::sleep_for(std::chrono::milliseconds(2));
+    }
+    check(hipEventSynchronize(ctx->ev[2]), "render");
+    DevCounters c{};
+    check(hipMemcpy(&c, P.counters, sizeof(c), hipMemcpyDeviceToHost), "read counters");
+    float t_trace = 0.0f, t_fin = 0.0f;
+    check(hipEventElapsedTime(&t_trace, ctx->ev[0], ctx->ev[1]), "event time");
+    check(hipEventElapsedTime(&t_fin, ctx->ev[1], ctx->ev[2]), "event time");
+    const bool cancelled = c.queue_head < static_cast<unsigned long long>(n_rays);
+    GRRTStats& s = ctx->last;
+    s = GRRTStats{};
+    s.trace_ms = t_trace;
+    s.finalize_ms = t_fin;
+    s.rays = static_cast<int64_t>(c.rays_done);
+    s.steps = static_cast<int64_t>(c.steps);
+    s.attempts = static_cast<int64_t>(c.attempts);
+    // one evaluation at each ray start plus six per attempt (k1 reused)
+    s.deriv_evals = static_cast<int64_t>(c.rays_done + 6 * c.attempts);
+    s.algo_flops = flops_per_attempt() * static_cast<double>(c.attempts);
+    s.cancelled = cancelled ? 1 : 0;
+    s.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    ctx->last_rays = n_rays;
+    ctx->records_valid = ctx->keep_records && !cancelled;
+    ctx->progress.store(1.0f);
+    if (progress) progress(1.0f, ud);
+    return cancelled ? 1 : 0;
+}
+
+void tonemap_host(float* fb, int w, int h) {
+    // Log-average auto-exposure with extended Reinhard and gamma 2.2
+    // (src/tonemapper.cpp:7-71).
+    const long long n = static_cast<long long>(w) * h;
+    const double key = 0.18, delta = 1e-6;
+    double log_sum = 0.0, lmax = 0.0;
+    long long lit = 0;
+    auto lum = [&](long long i) {
+        const double R = std::max(0.0, static_cast<double>(fb[4 * i]));
+        const double G = std::max(0.0, static_cast<double>(fb[4 * i + 1]));
+        const double B = std::max(0.0, static_cast<double>(fb[4 * i + 2]));
+        return 0.2126 * R + 0.7152 * G + 0.0722 * B;
+    };
+    for (long long i = 0; i < n; ++i) {
+        const double L = lum(i);
+        if (L > 1e-10) { log_sum += std::log(delta + L); ++lit; }
+        if (L > lmax) lmax = L;
+    }
+    if (lmax <= 0.0 || lit == 0) return;
+    const double exposure = key / std::exp(log_sum / lit);
+    const double lw = lmax * exposure;
+    const double lw2 = lw * lw;
+    auto gam = [](double v) { return std::pow(std::clamp(v, 0.0, 1.0), 1.0 / 2.2); };
+    for (long long i = 0; i < n; ++i) {
+        const double L = lum(i);
+        if (L <= 0.0) { fb[4 * i] = fb[4 * i + 1] = fb[4 * i + 2] = 0.0f; continue; }
+        const double ls = L * exposure;
+        const double lm = std::clamp(ls * (1.0 + ls / lw2) / (1.0 + ls), 0.0, 1.0);
+        const double sc = lm / L;
+        for (int c = 0; c < 3; ++c)
+            fb[4 * i + c] = static_cast<float>(gam(std::max(0.0, static_cast<double>(fb[4 * i + c])) * sc));
+    }
+}
+
+template <class F>
+int guarded(GRRTContext* ctx, F&& f) {
+    if (!ctx) { g_last_error = "null context"; return -1; }
+    try {
+        return f();
+    } catch (const std::exception& e) {
+        ctx->error_msg = e.what();
+        return -1;
+    } catch (...) {
+        ctx->error_msg = "unknown error";
+        return -1;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+GRRTContext* grrt_create(const GRRTParams* params) {
+    if (!params) { g_last_error = "grrt_create: params is null"; return nullptr; }
+    GRRTContext* ctx = nullptr;
+    try {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+            g_last_error = "no HIP device available (this library has only the GPU backend)";
+            return nullptr;
+        }
+        ctx = new GRRTContext();
+        ctx->params = *params;
+        check(hipGetDevice(&ctx->device), "hipGetDevice");
+        check(hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, ctx->device),
+              "hipDeviceGetAttribute");
+        check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+        for (auto& e : ctx->ev) check(hipEventCreate(&e), "hipEventCreate");
+        void* flag = nullptr;
+        check(hipHostMalloc(&flag, sizeof(int), hipHostMallocMapped | hipHostMallocCoherent), "hipHostMalloc");
+        ctx->cancel_flag = static_cast<volatile int*>(flag);
+        *ctx->cancel_flag = 0;
+        void* dflag = nullptr;
+        check(hipHostGetDevicePointer(&dflag, flag, 0), "hipHostGetDevicePointer");
+        ctx->cancel_flag_dev = static_cast<const volatile int*>(dflag);
+        build_scene(ctx);
+        return ctx;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+    } catch (...) {
+        g_last_error = "grrt_create: unknown error";
+    }
+    grrt_destroy(ctx);
+    return nullptr;
+}
+
+void grrt_destroy(GRRTContext* ctx) {
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    ctx->flux.release();
+    ctx->spec.release();
+    ctx->star_tpb.release();
+    ctx->star_start.release();
+    ctx->sample_rgb.release();
+    ctx->sample_cls.release();
+    ctx->records.release();
+    ctx->frame.release();
+    ctx->counters.release();
+    if (ctx->cancel_flag) (void)hipHostFree(const_cast<int*>(ctx->cancel_flag));
+    for (auto& e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+void grrt_update_params(GRRTContext* ctx, const GRRTParams* params) {
+    if (!ctx || !params) return;
+    guarded(ctx, [&] {
+        ctx->params = *params;
+        build_scene(ctx);
+        return 0;
+    });
+}
+
+int grrt_render(GRRTContext* ctx, float* framebuffer) {
+    return grrt_render_cb(ctx, framebuffer, nullptr, nullptr);
+}
+
+int grrt_render_cb(GRRTContext* ctx, float* framebuffer, grrt_progress_fn progress, void* user_data) {
+    return guarded(ctx, [&] {
+        if (!framebuffer) throw HipFailure("framebuffer is null");
+        const auto& r = ctx->res;
+        const size_t npix = static_cast<size_t>(r.W) * r.H;
+        check(ctx->frame.ensure(4 * npix), "hipMalloc(frame)");
+        const int rc = render_window(ctx, 0, 0, r.W, r.H, ctx->frame.ptr, progress, user_data);
+        check(hipMemcpy(framebuffer, ctx->frame.ptr, 4 * npix * sizeof(float), hipMemcpyDeviceToHost),
+              "copy frame");
+        return rc;
+    });
+}
+
+int grrt_render_tile(GRRTContext* ctx, float* buffer, int x, int y, int tile_width, int tile_height) {
+    return guarded(ctx, [&] {
+        const auto& r = ctx->res;
+        if (!buffer || tile_width <= 0 || tile_height <= 0 || x < 0 || y < 0 || x + tile_width > r.W ||
+            y + tile_height > r.H)
+            throw HipFailure("grrt_render_tile: tile outside the frame");
+        const size_t npix = static_cast<size_t>(tile_width) * tile_height;
+        check(ctx->frame.ensure(4 * npix), "hipMalloc(frame)");
+        const int rc = render_window(ctx, x, y, tile_width, tile_height, ctx->frame.ptr, nullptr, nullptr);
+        check(hipMemcpy(buffer, ctx->frame.ptr, 4 * npix * sizeof(float), hipMemcpyDeviceToHost),
+              "copy tile");
+        return rc;
+    });
+}
+
+void grrt_cancel(GRRTContext* ctx) {
+    if (ctx && ctx->cancel_flag) *ctx->cancel_flag = 1;
+}
+
+float grrt_progress(const GRRTContext* ctx) { return ctx ? ctx->progress.load() : 0.0f; }
+
+const char* grrt_error(const GRRTContext* ctx) {
+    return (!ctx || ctx->error_msg.empty()) ? nullptr : ctx->error_msg.c_str();
+}
+
+int grrt_cuda_available(void) {
+    int n = 0;
+    return (hipGetDeviceCount(&n) == hipSuccess && n > 0) ? 1 : 0;
+}
+
+const char* grrt_last_error(void) { return g_last_error.empty() ? nullptr : g_last_error.c_str(); }
+
+void grrt_tonemap(float* framebuffer, int width, int height) {
+    if (framebuffer && width > 0 && height > 0) tonemap_host(framebuffer, width, height);
+}
+
+void grrt_debug_pixel(GRRTContext* ctx, int px, int py) {
+    if (!ctx) { std::printf("grrt_debug_pixel: context not ready\n"); return; }
+    guarded(ctx, [&] {
+        const auto& r = ctx->res;
+        if (px < 0 || py < 0 || px >= r.W || py >= r.H) throw HipFailure("pixel outside the frame");
+        const int mode = ctx->sampling_mode;
+        const bool keep = ctx->keep_records;
+        ctx->sampling_mode = GRRT_SAMPLING_CENTER;
+        ctx->keep_records = true;
+        check(ctx->frame.ensure(4), "hipMalloc(frame)");
+        render_window(ctx, px, py, 1, 1, ctx->frame.ptr, nullptr, nullptr);
+        ctx->sampling_mode = mode;
+        ctx->keep_records = keep;
+        GRRTRayRecord rec{};
+        check(hipMemcpy(&rec, ctx->records.ptr, sizeof(rec), hipMemcpyDeviceToHost), "copy record");
+        static const char* names[] = {"HORIZON", "ESCAPED", "MAX STEPS"};
+        std::printf("Debugging pixel (%d, %d) in %dx%d image\n", px, py, r.W, r.H);
+        std::printf("  -> %s after %d steps (%d attempts)  r=%.6f theta=%.6f phi=%.6f\n",
+                    names[rec.term < 0 || rec.term > 2 ? 2 : rec.term], rec.steps, rec.attempts,
+                    rec.x_final[1], rec.x_final[2], rec.x_final[3]);
+        std::printf("     color=(%.4e %.4e %.4e) crossings=%d r_hit=%.6f g_hit=%.6f |H|max=%.3e\n",
+                    rec.color[0], rec.color[1], rec.color[2], rec.n_cross, rec.r_hit, rec.g_hit, rec.h_max);
+        return 0;
+    });
+}
+
+void grrt_set_frequency_bins(GRRTContext* ctx, const double* frequencies_hz, int num_bins) {
+    if (!ctx) return;
+    if (num_bins > 0 && frequencies_hz) ctx->frequency_bins.assign(frequencies_hz, frequencies_hz + num_bins);
+    else ctx->frequency_bins.clear();
+}
+
+int grrt_render_spectral(GRRTContext* ctx, double* spectral_buffer, int width, int height) {
+    return grrt_render_spectral_cb(ctx, spectral_buffer, width, height, nullptr, nullptr);
+}
+
+int grrt_render_spectral_cb(GRRTContext* ctx, double* spectral_buffer, int width, int height,
+                            grrt_progress_fn progress, void* user_data) {
+    return guarded(ctx, [&] {
+        if (ctx->frequency_bins.empty()) {
+            ctx->error_msg = "No frequency bins set -- call grrt_set_frequency_bins first";
+            return -1;
+        }
+        if (!spectral_buffer || width <= 0 || height <= 0) throw HipFailure("invalid spectral buffer");
+        if (!ctx->res.volumetric) {
+            // Without a volumetric disk every spectrum is zero
+            // (src/geodesic_tracer.cpp:503-505).
+            std::fill_n(spectral_buffer,
+                        static_cast<size_t>(width) * height * ctx->frequency_bins.size(), 0.0);
+            if (progress) progress(1.0f, user_data);
+            return 0;
+        }
+        throw HipFailure("the volumetric spectral path is not available in this build of the GPU backend");
+    });
+}
+
+int grrt_render_spectral_to_fits_cb(GRRTContext* ctx, const char* output_path, int width, int height,
+                                    grrt_progress_fn progress, void* user_data) {
+    (void)width;
+    (void)height;
+    (void)progress;
+    (void)user_data;
+    return guarded(ctx, [&] {
+        if (ctx->frequency_bins.empty()) {
+            ctx->error_msg = "No frequency bins set -- call grrt_set_frequency_bins first";
+            return -1;
+        }
+        if (!output_path || output_path[0] == '\0') {
+            ctx->error_msg = "grrt_render_spectral_to_fits_cb: output_path is null or empty";
+            return -1;
+        }
+        throw HipFailure("the FITS writer is not available in this build of the GPU backend");
+    });
+}
+
+// ------------------------------------------------------------ extensions ---
+
+int grrt_ext_set_sampling(GRRTContext* ctx, int mode, int spp, uint64_t seed) {
+    return guarded(ctx, [&] {
+        if (mode < GRRT_SAMPLING_HASH || mode > GRRT_SAMPLING_SEEDED) throw HipFailure("unknown sampling mode");
+        ctx->sampling_mode = mode;
+        ctx->sampling_spp = std::max(spp, 1);
+        ctx->sampling_seed = seed;
+        return 0;
+    });
+}
+
+int grrt_ext_set_records(GRRTContext* ctx, int enable) {
+    return guarded(ctx, [&] {
+        ctx->keep_records = enable != 0;
+        ctx->records_valid = false;
+        return 0;
+    });
+}
+
+int64_t grrt_ext_num_samples(const GRRTContext* ctx) {
+    if (!ctx) return -1;
+    int grid = 1;
+    const int nS = samples_per_pixel(ctx->sampling_mode, ctx->sampling_spp, grid);
+    return static_cast<int64_t>(ctx->res.W) * ctx->res.H * nS;
+}
+
+int grrt_ext_get_records(GRRTContext* ctx, GRRTRayRecord* out, int64_t n) {
+    return guarded(ctx, [&] {
+        if (!ctx->records_valid) throw HipFailure("no records: enable them before rendering");
+        if (!out || n < 0 || n > ctx->last_rays) throw HipFailure("record count out of range");
+        check(hipMemcpy(out, ctx->records.ptr, static_cast<size_t>(n) * sizeof(GRRTRayRecord),
+                        hipMemcpyDeviceToHost), "copy records");
+        return 0;
+    });
+}
+
+int grrt_ext_get_classes(GRRTContext* ctx, uint8_t* out, int64_t n) {
+    return guarded(ctx, [&] {
+        if (!out || n < 0 || n > ctx->last_rays) throw HipFailure("class count out of range");
+        check(hipMemcpy(out, ctx->sample_cls.ptr, static_cast<size_t>(n), hipMemcpyDeviceToHost),
+              "copy classes");
+        return 0;
+    });
+}
+
+int grrt_ext_last_stats(const GRRTContext* ctx, GRRTStats* out) {
+    if (!ctx || !out) return -1;
+    *out = ctx->last;
+    return 0;
+}
+
+int grrt_ext_render_device(GRRTContext* ctx, float* d_rgba) {
+    return guarded(ctx, [&] {
+        const auto& r = ctx->res;
+        float* dst = d_rgba;
+        if (!dst) {
+            check(ctx->frame.ensure(4 * static_cast<size_t>(r.W) * r.H), "hipMalloc(frame)");
+            dst = ctx->frame.ptr;
+        }
+        return render_window(ctx, 0, 0, r.W, r.H, dst, nullptr, nullptr);
+    });
+}
+
+int grrt_ext_set_tuning(GRRTContext* ctx, int block, int blocks_per_cu, int refill_min) {
+    return guarded(ctx, [&] {
+        if (block > 0) {
+            if (block % 64 != 0 || block > 256) throw HipFailure("block must be a multiple of 64, at most 256");
+            ctx->tune_block = block;
+        }
+        if (blocks_per_cu > 0) ctx->tune_blocks_per_cu = blocks_per_cu;
+        if (refill_min > 0) ctx->tune_refill = std::min(refill_min, 64);
+        return 0;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,85 @@
+// GRRTContext: the state behind the opaque handle of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "grrt/types.h"
+#include "grrt_ext.h"
+#include "scene_host.hpp"
+#include "trace_params.hpp"
+
+namespace grrt_amd {
+
+// Device buffer that grows on demand and is released with the context.
+template <class T>
+struct DevBuf {
+    T* ptr = nullptr;
+    size_t cap = 0;
+    hipError_t ensure(size_t n) {
+        if (n <= cap) return hipSuccess;
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&ptr), n * sizeof(T));
+        if (e == hipSuccess) cap = n;
+        return e;
+    }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+};
+
+hipError_t launch_reset(DevCounters* c, hipStream_t st);
+hipError_t launch_trace_thin(const TraceParams& P, int blocks, int threads, hipStream_t st);
+hipError_t launch_finalize(const double* rgb, float* out, long long npix, int nS,
+                           const DevCounters* cnt, hipStream_t st);
+double flops_per_attempt();
+
+}  // namespace grrt_amd
+
+struct GRRTContext {
+    GRRTParams params{};
+    grrt_amd::Resolved res{};
+    std::string error_msg;
+    std::vector<double> frequency_bins;
+
+    int device = 0;
+    int num_cus = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+
+    // Scene tables in device memory.
+    grrt_amd::CameraFrame cam{};
+    double flux_max = 0.0;
+    grrt_amd::DevBuf<double> flux, spec, star_tpb;
+    grrt_amd::DevBuf<int> star_start;
+
+    // Per-render buffers.
+    grrt_amd::DevBuf<double> sample_rgb;
+    grrt_amd::DevBuf<uint8_t> sample_cls;
+    grrt_amd::DevBuf<GRRTRayRecord> records;
+    grrt_amd::DevBuf<float> frame;
+    grrt_amd::DevBuf<grrt_amd::DevCounters> counters;
+    volatile int* cancel_flag = nullptr;     // host-mapped, polled by the kernel
+    const volatile int* cancel_flag_dev = nullptr;   // device alias of cancel_flag
+
+    // Sampling and tuning.
+    int sampling_mode = GRRT_SAMPLING_HASH;
+    int sampling_spp = 1;
+    uint64_t sampling_seed = 0;
+    bool keep_records = false;
+    bool records_valid = false;
+    int tune_block = 256;
+    int tune_blocks_per_cu = 2;
+    int tune_refill = 16;
+
+    GRRTStats last{};
+    long long last_rays = 0;
+    std::atomic<float> progress{1.0f};
+};

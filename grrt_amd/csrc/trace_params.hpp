@@ -1,0 +1,70 @@
+// Launch parameters of the thin-disk trace kernel, shared by host and device.
+// Passed by value as the kernel argument (well under the 4 KiB limit).
+#pragma once
+#include <cstdint>
+
+#include "grrt_ext.h"
+
+namespace grrt_amd {
+
+constexpr int kFluxN = 500;       // src/accretion_disk.h:163 default LUT size
+constexpr int kSpecN = 1000;      // src/spectrum.h:27 default entries
+constexpr double kSpecTmin = 1000.0, kSpecTmax = 100000.0;
+constexpr int kStarGT = 180, kStarGP = 360;   // src/celestial_sphere.h:123-124
+constexpr int kStarCells = kStarGT * kStarGP;
+
+// Queue and statistics counters in device memory (reset by a kernel before
+// every launch, so the launch sequence is graph-capturable).
+struct DevCounters {
+    unsigned long long queue_head;    // next unclaimed ray index
+    unsigned long long steps;
+    unsigned long long attempts;
+    unsigned long long derivs;
+    unsigned long long rays_done;
+    unsigned long long pad[3];
+};
+
+struct TraceParams {
+    // Kerr metric in Boyer–Lindquist coordinates.
+    double M, a;
+    double r_term;       // horizon radius + 0.01 (src/geodesic_tracer.h:64)
+    double r_escape;     // 1000 M (src/api.cpp:132)
+    double tol;
+    int max_steps;
+
+    // Camera: static observer tetrad and the covariant metric there.
+    int W, H;
+    double r_obs, th_obs, ph_obs, fov;
+    double e0t;                         // e0 = (e0t, 0, 0, 0)
+    double e1[4], e2[4], e3[4];
+    double g00, g03, g11, g22, g33;     // g_lower at the observer
+
+    // Sampling: ray index -> (pixel in window, sample).
+    int mode, grid, nS;
+    double cell;
+    uint64_t seed;
+    int x0, y0, tw, th;                 // pixel window (full frame or tile)
+    long long n_rays;
+
+    // Thin disk (src/accretion_disk.cpp).
+    int disk;
+    double r_in, r_out, t_peak, flux_max, ut_obs;
+    const double* flux_lut;             // kFluxN
+    const double* spec_lut;             // kSpecN * 3
+
+    // Star background (src/celestial_sphere.cpp), CSR over the 180x360 grid.
+    int stars;
+    const int* star_start;              // kStarCells + 1
+    const double* star_tpb;             // (theta, phi, brightness) in cell order
+
+    // Outputs.
+    double* sample_rgb;                 // n_rays * 3
+    uint8_t* sample_cls;                // n_rays
+    GRRTRayRecord* records;             // n_rays or nullptr
+    DevCounters* counters;
+    const volatile int* cancel;         // host-mapped flag, may be nullptr
+
+    int refill_min;                     // idle lanes that trigger a refill
+};
+
+}  // namespace grrt_amd

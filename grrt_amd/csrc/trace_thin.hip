@@ -1,0 +1,518 @@
+// Persistent thin-disk geodesic tracer for gfx950.
+//
+// One lane owns one ray.  A wavefront loops over Dormand–Prince attempts;
+// every loop iteration is exactly one attempt for every live lane, so lanes
+// whose rays take more or fewer steps never wait on each other.  When enough
+// lanes have finished (refill_min), the wavefront claims that many new ray
+// indices from a device-wide queue with a single atomic and hands them to
+// its idle lanes in lane order (ballot + mbcnt), which keeps the wavefront
+// full until the queue drains.
+//
+// Per-ray semantics follow the reference:
+//   src/geodesic_tracer.cpp:23-181   outer loop, termination, disk crossings
+//   src/rk4.cpp:279-328              adaptive step control
+//   include/grrt/geodesic/rk4.h:73-177  Dormand–Prince stages and error norm
+//   src/camera.cpp:69-96             pixel -> null covector
+//   src/renderer.cpp:14-82           sample placement and averaging
+//   src/accretion_disk.cpp:96-147    thin-disk emission
+//   src/celestial_sphere.cpp:48-90   star background
+// The accepted state's k7 (first-same-as-last) becomes the next step's k1,
+// which is the same evaluation the reference repeats.
+#include <hip/hip_runtime.h>
+
+#include "device_math.hpp"
+#include "kerr_rhs.hpp"
+#include "trace_params.hpp"
+
+namespace grrt_amd {
+
+// Dormand–Prince 5(4) tableau (Dormand & Prince 1980).
+constexpr double A21 = 1.0 / 5.0;
+constexpr double A31 = 3.0 / 40.0, A32 = 9.0 / 40.0;
+constexpr double A41 = 44.0 / 45.0, A42 = -56.0 / 15.0, A43 = 32.0 / 9.0;
+constexpr double A51 = 19372.0 / 6561.0, A52 = -25360.0 / 2187.0, A53 = 64448.0 / 6561.0,
+                 A54 = -212.0 / 729.0;
+constexpr double A61 = 9017.0 / 3168.0, A62 = -355.0 / 33.0, A63 = 46732.0 / 5247.0,
+                 A64 = 49.0 / 176.0, A65 = -5103.0 / 18656.0;
+constexpr double B1 = 35.0 / 384.0, B3 = 500.0 / 1113.0, B4 = 125.0 / 192.0,
+                 B5 = -2187.0 / 6784.0, B6 = 11.0 / 84.0;
+constexpr double E1 = 71.0 / 57600.0, E3 = -71.0 / 16695.0, E4 = 71.0 / 1920.0,
+                 E5 = -17253.0 / 339200.0, E6 = 22.0 / 525.0, E7 = -1.0 / 40.0;
+
+// Algorithmic FP64 work per Dormand–Prince attempt as this kernel performs
+// it: six right-hand-side evaluations (k1 is reused) of 120 flops each, 26
+// coefficient products, stage/solution/error combinations over five
+// components (5*(2+4+6+8+10) + 5*10 + 5*12 = 260), the error norm (5*5) and
+// the step controller (12).  Used only for reporting.
+constexpr double kFlopsPerAttempt = 6 * 120 + 26 + 260 + 25 + 12;
+
+// 0.9 * x^(1/5) for the step controller, in FP32.
+__device__ __forceinline__ double step_factor(double x) {
+    const float lx = __builtin_amdgcn_logf(static_cast<float>(x));   // log2
+    return static_cast<double>(0.9f * __builtin_amdgcn_exp2f(0.2f * lx));
+}
+
+struct Ray {
+    Vec5 y;          // r, theta, phi, p_r, p_theta
+    Vec5 k1;         // right-hand side at y
+    double P0, P3;   // conserved p_t, p_phi
+    double dl;
+    double col[3];
+    double r_hit, g_hit, h_max, h_last;
+    int steps, attempts, retry, n_cross;
+};
+
+__device__ __forceinline__ double hamiltonian_from_rhs(const Ray& R, double dt) {
+    // H = p_mu dx^mu / 2 with dx^mu = g^{mu nu} p_nu.
+    return 0.5 * (R.P0 * dt + R.y.v[3] * R.k1.v[0] + R.y.v[4] * R.k1.v[1] + R.P3 * R.k1.v[2]);
+}
+
+// Fractional pixel position of ray `id` (src/renderer.cpp:48-55 and the
+// extension sampling modes).
+__device__ __forceinline__ void sample_position(const TraceParams& P, long long id, double& px,
+                                                double& py) {
+    const long long pl = id / P.nS;
+    const int s = static_cast<int>(id - pl * P.nS);
+    const int i = P.x0 + static_cast<int>(pl % P.tw);
+    const int j = P.y0 + static_cast<int>(pl / P.tw);
+    if (P.mode == GRRT_SAMPLING_CENTER) {
+        px = i + 0.5;
+        py = j + 0.5;
+    } else if (P.mode == GRRT_SAMPLING_HASH) {
+        auto hash = [](int ii, int jj, int ss, int ch) -> double {
+            uint32_t h = static_cast<uint32_t>(ii * 73856093u ^ jj * 19349663u ^ ss * 83492791u ^
+                                               ch * 45678917u);
+            h ^= h >> 16;
+            h *= 0x45d9f3bu;
+            h ^= h >> 16;
+            return (h & 0xFFFFu) / 65536.0;
+        };
+        const int sy = s / P.grid, sx = s - sy * P.grid;
+        px = i + (sx + hash(i, j, s, 0)) * P.cell;
+        py = j + (sy + hash(i, j, s, 1)) * P.cell;
+    } else {
+        // Seeded: pixel-major global sample index over the full frame.
+        const long long gid = (static_cast<long long>(j) * P.W + i) * P.nS + s;
+        const uint64_t key = P.seed * 0xD1B54A32D192ED03ull;
+        const uint64_t w = 0x9E3779B97F4A7C15ull;
+        const uint64_t u = static_cast<uint64_t>(gid) * 2u;
+        const double jx = static_cast<double>(mix64(key + (u + 1u) * w) >> 11) * 0x1.0p-53;
+        const double jy = static_cast<double>(mix64(key + (u + 2u) * w) >> 11) * 0x1.0p-53;
+        px = i + jx;
+        py = j + jy;
+    }
+}
+
+// Camera ray (src/camera.cpp:69-96): p^mu = -e0 + d, lowered with g at the
+// observer; only the non-zero metric entries enter.
+__device__ __forceinline__ void init_ray(const TraceParams& P, const KerrK& K, long long id, Ray& R) {
+    double px, py;
+    sample_position(P, id, px, py);
+    const double alpha = (px - P.W / 2.0) * P.fov / P.W;
+    const double beta = (py - P.H / 2.0) * P.fov / P.W;
+    double sa, ca_, sb, cb;
+    sincos(alpha, &sa, &ca_);
+    sincos(beta, &sb, &cb);
+    const double ca = cb * sa, cc = cb * ca_;
+    double pc[4];
+    pc[0] = -P.e0t + (-ca * P.e1[0] - sb * P.e2[0] + cc * P.e3[0]);
+#pragma unroll
+    for (int m = 1; m < 4; ++m) pc[m] = -ca * P.e1[m] - sb * P.e2[m] + cc * P.e3[m];
+    R.P0 = P.g00 * pc[0] + P.g03 * pc[3];
+    R.P3 = P.g03 * pc[0] + P.g33 * pc[3];
+    R.y.v[0] = P.r_obs;
+    R.y.v[1] = P.th_obs;
+    R.y.v[2] = P.ph_obs;
+    R.y.v[3] = P.g11 * pc[1];
+    R.y.v[4] = P.g22 * pc[2];
+    R.dl = 0.01 * P.r_obs;
+    R.col[0] = R.col[1] = R.col[2] = 0.0;
+    R.r_hit = R.g_hit = 0.0;
+    R.steps = R.attempts = R.retry = R.n_cross = 0;
+    double dt;
+    R.k1 = kerr_rhs(K, R.P0, R.P3, R.y, dt);
+    R.h_last = __builtin_fabs(hamiltonian_from_rhs(R, dt));
+    R.h_max = R.h_last;
+}
+
+// Thin-disk emission at an in-range crossing (src/accretion_disk.cpp:96-147).
+__device__ __forceinline__ void disk_emission(const TraceParams& P, const double* s_flux,
+                                              const double* s_spec, double rc, double P0,
+                                              double P3, Ray& R) {
+    double F = 0.0;
+    if (!(rc <= P.r_in || rc >= P.r_out || P.flux_max <= 0.0)) {
+        const double frac = (rc - P.r_in) / (P.r_out - P.r_in) * (kFluxN - 1);
+        const int idx = static_cast<int>(frac);
+        const double t = frac - idx;
+        F = (idx >= kFluxN - 1) ? s_flux[kFluxN - 1] : s_flux[idx] * (1.0 - t) + s_flux[idx + 1] * t;
+    }
+    // Redshift g = (p.u)_emit / (p.u)_obs for a circular emitter and the
+    // static observer (src/accretion_disk.cpp:113-130).
+    const double w = sqrt(P.M / (rc * rc * rc));
+    const double aw = P.a * w;
+    const double ut_e = 1.0 / sqrt(1.0 - 3.0 * P.M / rc + 2.0 * aw);
+    const double up_e = (w / (1.0 + P.a * w)) * ut_e;
+    const double pu_obs = P0 * P.ut_obs;
+    const double pu_e = P0 * ut_e + P3 * up_e;
+    const double g = (__builtin_fabs(pu_obs) < 1e-30) ? 1.0 : pu_e / pu_obs;
+    if (R.n_cross == 0) {
+        R.r_hit = rc;
+        R.g_hit = g;
+    }
+    ++R.n_cross;
+    if (F <= 0.0 || P.flux_max <= 0.0) return;
+    const double T = P.t_peak * sqrt(sqrt(F / P.flux_max));
+    if (T <= 0.0) return;
+    const double Tobs = g * T;
+    if (!(Tobs >= 100.0)) return;       // also drops a NaN temperature
+    // Chromaticity lerp (src/spectrum.cpp:321-330); luminosity (T/1e5)^4.
+    const double Tc = sclamp(Tobs, kSpecTmin, kSpecTmax);
+    const double frac = (Tc - kSpecTmin) / (kSpecTmax - kSpecTmin) * (kSpecN - 1);
+    const int idx = static_cast<int>(frac);
+    const double t = frac - idx;
+    const double ratio = T / kSpecTmax;
+    const double lum = ratio * ratio * ratio * ratio;
+    const double g4 = g * g * g * g;
+    const double wgt = lum * g4;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        const double cv = (idx >= kSpecN - 1)
+                              ? s_spec[3 * (kSpecN - 1) + ch]
+                              : s_spec[3 * idx + ch] * (1.0 - t) + s_spec[3 * (idx + 1) + ch] * t;
+        R.col[ch] += cv * wgt;
+    }
+}
+
+// Nearest star within 0.003 rad in the 3x3 neighbouring cells, first in
+// cell-then-catalogue order (src/celestial_sphere.cpp:48-90).
+__device__ double star_sample(const TraceParams& P, double theta, double phi) {
+    const double twopi = 2.0 * kPi;
+    theta = fmod(theta, twopi);
+    if (theta < 0.0) theta += twopi;
+    if (theta > kPi) {
+        theta = twopi - theta;
+        phi += kPi;
+    }
+    phi = fmod(phi, twopi);
+    if (phi > kPi) phi -= twopi;
+    if (phi < -kPi) phi += twopi;
+    const int tb = static_cast<int>(theta * kStarGT / kPi);
+    const int pb = static_cast<int>((phi + kPi) * kStarGP / twopi);
+    const double st = sin(theta);
+    for (int dt = -1; dt <= 1; ++dt) {
+        int t = tb + dt;
+        t = t < 0 ? 0 : (t > kStarGT - 1 ? kStarGT - 1 : t);
+        for (int dp = -1; dp <= 1; ++dp) {
+            const int p = (((pb + dp) % kStarGP) + kStarGP) % kStarGP;
+            const int cell = t * kStarGP + p;
+            const int e = P.star_start[cell + 1];
+            for (int k = P.star_start[cell]; k < e; ++k) {
+                const double sth = P.star_tpb[3 * k], sph = P.star_tpb[3 * k + 1];
+                const double dth = theta - sth;
+                double dph = phi - sph;
+                if (dph > kPi) dph -= twopi;
+                if (dph < -kPi) dph += twopi;
+                const double d2 = dth * dth + dph * dph * st * st;
+                if (d2 < 0.003 * 0.003) return P.star_tpb[3 * k + 2];
+            }
+        }
+    }
+    return 0.0;
+}
+
+__device__ __forceinline__ void finish_ray(const TraceParams& P, long long id, Ray& R, int term) {
+    if (term == 1 && P.stars) {
+        const double b = star_sample(P, R.y.v[1], R.y.v[2]);
+        R.col[0] += b;
+        R.col[1] += b;
+        R.col[2] += b;
+    }
+    const int cls = (R.n_cross > 0) ? GRRT_CLASS_DISK
+                                    : (term == 0 ? GRRT_CLASS_CAPTURED
+                                                 : (term == 1 ? GRRT_CLASS_ESCAPED : GRRT_CLASS_MAXSTEPS));
+    P.sample_rgb[3 * id + 0] = R.col[0];
+    P.sample_rgb[3 * id + 1] = R.col[1];
+    P.sample_rgb[3 * id + 2] = R.col[2];
+    P.sample_cls[id] = static_cast<uint8_t>(cls);
+    if (P.records) {
+        GRRTRayRecord& o = P.records[id];
+        o.color[0] = R.col[0];
+        o.color[1] = R.col[1];
+        o.color[2] = R.col[2];
+        o.x_final[0] = 0.0;
+        o.x_final[1] = R.y.v[0];
+        o.x_final[2] = R.y.v[1];
+        o.x_final[3] = R.y.v[2];
+        o.p_final[0] = R.P0;
+        o.p_final[1] = R.y.v[3];
+        o.p_final[2] = R.y.v[4];
+        o.p_final[3] = R.P3;
+        o.r_hit = R.r_hit;
+        o.g_hit = R.g_hit;
+        o.h_max = R.h_max;
+        o.h_final = R.h_last;
+        o.term = term;
+        o.cls = cls;
+        o.steps = R.steps;
+        o.attempts = R.attempts;
+        o.n_cross = R.n_cross;
+        o.pad = 0;
+    }
+}
+
+// Termination test of src/geodesic_tracer.cpp:52-63 on the current state;
+// -1 while the ray continues.
+__device__ __forceinline__ int termination(const TraceParams& P, const Ray& R) {
+    if (R.steps >= P.max_steps) return 2;
+    const double r = R.y.v[0];
+    if (r < P.r_term) return 0;
+    if (r > P.r_escape) return 1;
+    return -1;
+}
+
+__device__ __forceinline__ unsigned lane_id() {
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+__device__ __forceinline__ unsigned lanes_below(unsigned long long mask) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<unsigned>(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), 0u));
+}
+
+__global__ void reset_counters(DevCounters* c) {
+    if (threadIdx.x < sizeof(DevCounters) / 8)
+        reinterpret_cast<unsigned long long*>(c)[threadIdx.x] = 0ull;
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void trace_thin(TraceParams P) {
+    __shared__ double s_flux[kFluxN];
+    __shared__ double s_spec[3 * kSpecN];
+    if (P.disk) {
+        for (int i = threadIdx.x; i < kFluxN; i += blockDim.x) s_flux[i] = P.flux_lut[i];
+        for (int i = threadIdx.x; i < 3 * kSpecN; i += blockDim.x) s_spec[i] = P.spec_lut[i];
+    }
+    __syncthreads();
+
+    const KerrK K{P.M, P.a, P.a * P.a, 2.0 * P.M};
+    const unsigned lane = lane_id();
+    Ray R;
+    long long id = -1;
+    bool live = false;
+    bool queue_open = true;
+    unsigned long long n_steps = 0, n_att = 0, n_rays = 0;
+    unsigned refills = 0;
+
+    while (true) {
+        // ---- refill idle lanes from the ray queue (wave-uniform branch) ----
+        const unsigned long long idle = __ballot(!live);
+        const int n_idle = __popcll(idle);
+        if (queue_open && (n_idle >= P.refill_min || n_idle == 64)) {
+            const int leader = __ffsll(static_cast<long long>(idle)) - 1;
+            unsigned long long base = 0;
+            if (lane == static_cast<unsigned>(leader)) {
+                // A cancelled render claims nothing more; every claimed ray completes.
+                const bool stop = P.cancel && (refills & 7u) == 0u && *P.cancel != 0;
+                base = stop ? (~0ull >> 1)
+                            : atomicAdd(&P.counters->queue_head, static_cast<unsigned long long>(n_idle));
+            }
+            base = __shfl(base, leader);
+            ++refills;
+            if (base + n_idle >= static_cast<unsigned long long>(P.n_rays)) queue_open = false;
+            if (!live) {
+                const unsigned long long mine = base + lanes_below(idle);
+                if (mine < static_cast<unsigned long long>(P.n_rays)) {
+                    id = static_cast<long long>(mine);
+                    init_ray(P, K, id, R);
+                    live = true;
+                    const int term = termination(P, R);
+                    if (term >= 0) {
+                        finish_ray(P, id, R, term);
+                        live = false;
+                        ++n_rays;
+                    }
+                }
+            }
+        }
+        if (__ballot(live) == 0ull) {
+            if (!queue_open) break;
+            continue;
+        }
+        if (!live) continue;
+
+        // ---- one Dormand–Prince attempt (include/grrt/geodesic/rk4.h:73-177) ----
+        const double h = R.dl;
+        const double h21 = h * A21;
+        const double h31 = h * A31, h32 = h * A32;
+        const double h41 = h * A41, h42 = h * A42, h43 = h * A43;
+        const double h51 = h * A51, h52 = h * A52, h53 = h * A53, h54 = h * A54;
+        const double h61 = h * A61, h62 = h * A62, h63 = h * A63, h64 = h * A64, h65 = h * A65;
+        Vec5 s, k2, k3, k4, k5, k6, k7, y5, e;
+        double dt7;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) s.v[i] = __builtin_fma(R.k1.v[i], h21, R.y.v[i]);
+        k2 = kerr_rhs(K, R.P0, R.P3, s, dt7);
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+            s.v[i] = __builtin_fma(k2.v[i], h32, __builtin_fma(R.k1.v[i], h31, R.y.v[i]));
+        k3 = kerr_rhs(K, R.P0, R.P3, s, dt7);
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+            s.v[i] = __builtin_fma(k3.v[i], h43, __builtin_fma(k2.v[i], h42,
+                         __builtin_fma(R.k1.v[i], h41, R.y.v[i])));
+        k4 = kerr_rhs(K, R.P0, R.P3, s, dt7);
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+            s.v[i] = __builtin_fma(k4.v[i], h54, __builtin_fma(k3.v[i], h53,
+                         __builtin_fma(k2.v[i], h52, __builtin_fma(R.k1.v[i], h51, R.y.v[i]))));
+        k5 = kerr_rhs(K, R.P0, R.P3, s, dt7);
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+            s.v[i] = __builtin_fma(k5.v[i], h65, __builtin_fma(k4.v[i], h64,
+                         __builtin_fma(k3.v[i], h63, __builtin_fma(k2.v[i], h62,
+                             __builtin_fma(R.k1.v[i], h61, R.y.v[i])))));
+        k6 = kerr_rhs(K, R.P0, R.P3, s, dt7);
+        const double b1 = h * B1, b3 = h * B3, b4 = h * B4, b5 = h * B5, b6 = h * B6;
+        const double e1 = h * E1, e3 = h * E3, e4 = h * E4, e5 = h * E5, e6 = h * E6, e7 = h * E7;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            y5.v[i] = __builtin_fma(k6.v[i], b6, __builtin_fma(k5.v[i], b5,
+                          __builtin_fma(k4.v[i], b4, __builtin_fma(k3.v[i], b3,
+                              __builtin_fma(R.k1.v[i], b1, R.y.v[i])))));
+            e.v[i] = __builtin_fma(k6.v[i], e6, __builtin_fma(k5.v[i], e5,
+                         __builtin_fma(k4.v[i], e4, __builtin_fma(k3.v[i], e3, R.k1.v[i] * e1))));
+        }
+        k7 = kerr_rhs(K, R.P0, R.P3, y5, dt7);
+        double err = 0.0;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            const double ei = __builtin_fma(k7.v[i], e7, e.v[i]);
+            err = smax(err, __builtin_fabs(ei) * rcp(__builtin_fabs(y5.v[i]) + 1e-10));
+        }
+        ++R.attempts;
+
+        // ---- step control (src/rk4.cpp:279-328) ----
+        // The growth/shrink factor 0.9 (tol/err)^(1/5) is evaluated in FP32
+        // (hardware log2/exp2).  It only sets the next trial step length; a
+        // relative change d in that length moves the solution by about
+        // d * (local error), far below the FP64 rounding of the state.
+        bool accept;
+        double next;
+        if (R.retry >= 20) {
+            accept = true;           // retries exhausted: take the last attempt
+            next = h;
+        } else if (err <= P.tol) {
+            accept = true;
+            double f = (err > 1e-30) ? step_factor(P.tol / err) : 5.0;
+            f = sclamp(f, 1.0, 5.0);
+            next = smin(h * f, 5.0 * smax(__builtin_fabs(y5.v[0]), 1.0));
+        } else {
+            double f = step_factor(P.tol / err);
+            f = sclamp(f, 0.2, 1.0);
+            const double dl = h * f;
+            if (dl < 1e-6) {
+                accept = true;       // step too small: keep the rejected y5
+                next = smin(1e-6, 5.0 * smax(__builtin_fabs(y5.v[0]), 1.0));
+            } else {
+                accept = false;
+                next = dl;
+            }
+        }
+        R.dl = next;
+        if (!accept) {
+            ++R.retry;
+            continue;
+        }
+        R.retry = 0;
+
+        // ---- thin-disk crossing between the previous and new state ----
+        if (P.disk) {
+            const double half_pi = kPi / 2.0;
+            const double dp = R.y.v[1] - half_pi, dn = y5.v[1] - half_pi;
+            if (dp * dn < 0.0 && __builtin_fabs(dp - dn) > 1e-12) {
+                const double frac = -dp / (dn - dp);
+                const double rc = R.y.v[0] + frac * (y5.v[0] - R.y.v[0]);
+                if (rc >= P.r_in && rc <= P.r_out)
+                    disk_emission(P, s_flux, s_spec, rc, R.P0, R.P3, R);
+            }
+        }
+        R.y = y5;
+        R.k1 = k7;
+        ++R.steps;
+        R.h_last = __builtin_fabs(hamiltonian_from_rhs(R, dt7));
+        R.h_max = smax(R.h_max, R.h_last);
+        const int term = termination(P, R);
+        if (term >= 0) {
+            n_steps += R.steps;
+            n_att += R.attempts;
+            ++n_rays;
+            finish_ray(P, id, R, term);
+            live = false;
+        }
+    }
+
+    // Per-wave statistics, one atomic per counter per wave.
+    for (int off = 32; off > 0; off >>= 1) {
+        n_steps += __shfl_xor(n_steps, off);
+        n_att += __shfl_xor(n_att, off);
+        n_rays += __shfl_xor(n_rays, off);
+    }
+    if (lane == 0) {
+        atomicAdd(&P.counters->steps, n_steps);
+        atomicAdd(&P.counters->attempts, n_att);
+        atomicAdd(&P.counters->rays_done, n_rays);
+    }
+}
+
+// Average the samples of each pixel in sample order (src/renderer.cpp:56-72)
+// and write RGBA float32 for the window.  Pixels with a sample beyond the
+// last claimed ray (a cancelled render) are written as zero.
+__global__ void finalize_pixels(const double* __restrict__ rgb, float* __restrict__ out,
+                                long long npix, int nS, double inv, const DevCounters* cnt) {
+    const long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    const unsigned long long done = cnt->queue_head;
+    float4 o;
+    if (static_cast<unsigned long long>((p + 1) * nS) > done) {
+        o.x = o.y = o.z = 0.0f;
+    } else {
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        for (int s = 0; s < nS; ++s) {
+            const double* c = rgb + 3 * (p * nS + s);
+            a0 = a0 + c[0];
+            a1 = a1 + c[1];
+            a2 = a2 + c[2];
+        }
+        o.x = static_cast<float>(a0 * inv);
+        o.y = static_cast<float>(a1 * inv);
+        o.z = static_cast<float>(a2 * inv);
+    }
+    o.w = 1.0f;
+    reinterpret_cast<float4*>(out)[p] = o;
+}
+
+}  // namespace grrt_amd
+
+// Host-callable launchers (C++ linkage inside the library).
+namespace grrt_amd {
+
+hipError_t launch_reset(DevCounters* c, hipStream_t st) {
+    hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, st, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace_thin(const TraceParams& P, int blocks, int threads, hipStream_t st) {
+    hipLaunchKernelGGL(trace_thin, dim3(blocks), dim3(threads), 0, st, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(const double* rgb, float* out, long long npix, int nS,
+                           const DevCounters* cnt, hipStream_t st) {
+    const int T = 256;
+    const long long B = (npix + T - 1) / T;
+    hipLaunchKernelGGL(finalize_pixels, dim3(static_cast<unsigned>(B)), dim3(T), 0, st, rgb, out,
+                       npix, nS, 1.0 / nS, cnt);
+    return hipGetLastError();
+}
+
+double flops_per_attempt() { return kFlopsPerAttempt; }
+
+}  // namespace grrt_amd

@@ -1,0 +1,78 @@
+/*
+ * Additive extension entry points of libgrrt.so.  None of these exists in
+ * the reference ABI; the 16 reference symbols in grrt/api.h are unchanged.
+ * They expose what parity checks and measurement need: the sample
+ * placement, per-ray records and outcome classes, device-resident output
+ * and per-render counters.
+ */
+#ifndef GRRT_EXT_H
+#define GRRT_EXT_H
+
+#include <stdint.h>
+#include "grrt/api.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Sample placement.
+ *  HASH   — the reference renderer's stratified jitter (src/renderer.cpp:14-20,
+ *           :48-55): grid = floor(sqrt(spp)), grid^2 samples.  The default.
+ *  CENTER — one ray through each pixel centre (src/camera.cpp:65-67).
+ *  SEEDED — spp samples at (i + jx, j + jy) with (jx, jy) uniform in [0,1)^2
+ *           from a counter-based generator keyed by (seed, sample index):
+ *           SplitMix64 finaliser over a Weyl sequence, jx from draw 2k+1,
+ *           jy from draw 2k+2, 53-bit mantissas. */
+enum { GRRT_SAMPLING_HASH = 0, GRRT_SAMPLING_CENTER = 1, GRRT_SAMPLING_SEEDED = 2 };
+
+/* Outcome class of a ray: a disk hit is any equatorial crossing with
+ * r_in <= r <= r_out; otherwise the termination decides. */
+enum { GRRT_CLASS_CAPTURED = 0, GRRT_CLASS_ESCAPED = 1, GRRT_CLASS_DISK = 2,
+       GRRT_CLASS_MAXSTEPS = 3 };
+
+/* Per-ray record (144 bytes).  x_final[0] (coordinate time) is not
+ * integrated and reads 0.  h_max / h_final are |H| = |g^{ab}p_a p_b|/2 over
+ * accepted states and at the end. */
+typedef struct {
+    double color[3];
+    double x_final[4];
+    double p_final[4];
+    double r_hit, g_hit;   /* first in-range disk crossing, 0 if none */
+    double h_max, h_final;
+    int32_t term;          /* 0 horizon, 1 escaped, 2 max steps */
+    int32_t cls;
+    int32_t steps, attempts, n_cross, pad;
+} GRRTRayRecord;
+
+typedef struct {
+    double trace_ms;       /* trace kernel(s), HIP events on the render stream */
+    double finalize_ms;    /* per-pixel sample reduction kernel */
+    double total_ms;       /* whole call, host wall clock, incl. copies */
+    int64_t rays;
+    int64_t steps;         /* accepted Dormand–Prince steps */
+    int64_t attempts;      /* Dormand–Prince attempts (accepted + rejected) */
+    int64_t deriv_evals;   /* Kerr right-hand-side evaluations */
+    double algo_flops;     /* algorithmic FP64 flops of the trace kernel */
+    int32_t cancelled;
+    int32_t pad;
+} GRRTStats;
+
+GRRT_EXPORT int grrt_ext_set_sampling(GRRTContext* ctx, int mode, int spp, uint64_t seed);
+/* Keep per-ray records of subsequent renders (enable=1) for grrt_ext_get_records. */
+GRRT_EXPORT int grrt_ext_set_records(GRRTContext* ctx, int enable);
+GRRT_EXPORT int64_t grrt_ext_num_samples(const GRRTContext* ctx);
+GRRT_EXPORT int grrt_ext_get_records(GRRTContext* ctx, GRRTRayRecord* out, int64_t n);
+GRRT_EXPORT int grrt_ext_get_classes(GRRTContext* ctx, uint8_t* out, int64_t n);
+GRRT_EXPORT int grrt_ext_last_stats(const GRRTContext* ctx, GRRTStats* out);
+/* Render the full frame into device memory (d_rgba: width*height*4 floats,
+ * or NULL to keep it in the context).  No host copy is made. */
+GRRT_EXPORT int grrt_ext_render_device(GRRTContext* ctx, float* d_rgba);
+/* Persistent-kernel launch shape: threads per block (multiple of 64),
+ * blocks per CU, and the number of idle lanes (1..64) that triggers a
+ * wavefront refill from the ray queue.  0 keeps the current value. */
+GRRT_EXPORT int grrt_ext_set_tuning(GRRTContext* ctx, int block, int blocks_per_cu, int refill_min);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

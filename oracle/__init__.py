@@ -1,0 +1,250 @@
+"""ctypes loader for the CPU oracle (test infrastructure only).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+``cpu_baseline`` leg import this package, and only as the checker.  The
+product library ``libgrrt.so`` never links or calls it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIBS: dict[str, C.CDLL] = {}
+
+
+class OracleScene(C.Structure):
+    _fields_ = [
+        ("width", C.c_int), ("height", C.c_int),
+        ("mass", C.c_double), ("spin_a", C.c_double),
+        ("r_obs", C.c_double), ("theta_obs", C.c_double), ("phi_obs", C.c_double),
+        ("fov", C.c_double),
+        ("disk_enabled", C.c_int),
+        ("r_in", C.c_double), ("r_out", C.c_double), ("t_peak", C.c_double),
+        ("stars", C.c_int),
+        ("tol", C.c_double), ("max_steps", C.c_int),
+        ("r_escape", C.c_double),
+    ]
+
+
+class OracleRay(C.Structure):
+    _fields_ = [
+        ("color", C.c_double * 3), ("x_final", C.c_double * 4), ("p_final", C.c_double * 4),
+        ("r_hit", C.c_double), ("g_hit", C.c_double),
+        ("h_max", C.c_double), ("h_final", C.c_double),
+        ("term", C.c_int32), ("cls", C.c_int32),
+        ("steps", C.c_int32), ("attempts", C.c_int32), ("n_cross", C.c_int32), ("pad", C.c_int32),
+    ]
+
+
+RAY_DTYPE = np.dtype([
+    ("color", "<f8", 3), ("x_final", "<f8", 4), ("p_final", "<f8", 4),
+    ("r_hit", "<f8"), ("g_hit", "<f8"), ("h_max", "<f8"), ("h_final", "<f8"),
+    ("term", "<i4"), ("cls", "<i4"), ("steps", "<i4"), ("attempts", "<i4"),
+    ("n_cross", "<i4"), ("pad", "<i4"),
+])
+assert RAY_DTYPE.itemsize == C.sizeof(OracleRay)
+
+JITTER_HASH, JITTER_CENTER, JITTER_SEEDED = 0, 1, 2
+
+_D = C.c_double
+_PD = C.POINTER(C.c_double)
+
+
+def build() -> None:
+    """Compile both oracle variants with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib(variant: str = "strict") -> C.CDLL:
+    name = "liboracle.so" if variant == "strict" else "liboracle_fast.so"
+    if name in _LIBS:
+        return _LIBS[name]
+    path = os.path.join(_HERE, "lib", name)
+    if not os.path.exists(path):
+        build()
+    L = C.CDLL(path)
+    L.oracle_fast_sincos.argtypes = [_D, _PD, _PD]
+    L.oracle_kerr_derivs.argtypes = [_D, _D, _PD, _PD, _PD, _PD]
+    L.oracle_g_lower.argtypes = [_D, _D, _PD, _PD]
+    L.oracle_hamiltonian.argtypes = [_D, _D, _PD, _PD]
+    L.oracle_hamiltonian.restype = _D
+    L.oracle_horizon.argtypes = [_D, _D]
+    L.oracle_horizon.restype = _D
+    L.oracle_isco.argtypes = [_D, _D]
+    L.oracle_isco.restype = _D
+    L.oracle_dp45_attempt.argtypes = [_D, _D, _PD, _D, _PD, _PD]
+    L.oracle_rk4_step.argtypes = [_D, _D, _PD, _D, _PD]
+    L.oracle_adaptive_step.argtypes = [_D, _D, _PD, _D, _D, _PD, C.POINTER(C.c_int)]
+    L.oracle_adaptive_step.restype = _D
+    L.oracle_pixel_hash.argtypes = [C.c_int] * 4
+    L.oracle_pixel_hash.restype = _D
+    L.oracle_seeded_jitter.argtypes = [C.c_uint64, C.c_int64, _PD, _PD]
+    L.oracle_camera_ray.argtypes = [C.POINTER(OracleScene), _D, _D, _PD]
+    L.oracle_flux_lut.argtypes = [C.POINTER(OracleScene), _PD, _PD]
+    L.oracle_spectrum_lut.argtypes = [_PD]
+    L.oracle_stars.argtypes = [_PD]
+    L.oracle_star_sample.argtypes = [_D, _D, _PD]
+    L.oracle_disk_emission.argtypes = [C.POINTER(OracleScene), _D, _D, _D, _PD, _PD, _PD]
+    L.oracle_trace.argtypes = [C.POINTER(OracleScene), _PD, C.POINTER(OracleRay)]
+    L.oracle_trace_samples.argtypes = [C.POINTER(OracleScene), C.c_int, C.c_int, C.c_uint64,
+                                       C.POINTER(C.c_int64), C.c_int64, C.c_void_p, C.c_int]
+    L.oracle_render.argtypes = [C.POINTER(OracleScene), C.c_int, C.c_int, C.c_uint64,
+                                C.POINTER(C.c_float), C.POINTER(C.c_uint8), C.c_int]
+    L.oracle_render.restype = C.c_int64
+    _LIBS[name] = L
+    return L
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(_PD)
+
+
+def scene(width=256, height=256, mass=1.0, spin_a=0.0, r_obs=50.0, theta_obs=1.3962634,
+          phi_obs=0.0, fov=1.5707963, disk=True, r_in=None, r_out=20.0, t_peak=1e7,
+          stars=False, tol=1e-8, max_steps=10000, r_escape=1000.0) -> OracleScene:
+    if r_in is None:
+        r_in = lib().oracle_isco(mass, spin_a)
+    return OracleScene(width, height, mass, spin_a, r_obs, theta_obs, phi_obs, fov,
+                       1 if disk else 0, r_in, r_out, t_peak, 1 if stars else 0,
+                       tol, max_steps, r_escape)
+
+
+def fast_sincos(x: float, variant="strict"):
+    s, c = C.c_double(), C.c_double()
+    lib(variant).oracle_fast_sincos(x, C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def kerr_derivs(M, a, x, p, variant="strict"):
+    x = np.ascontiguousarray(x, np.float64)
+    p = np.ascontiguousarray(p, np.float64)
+    dx, dp = np.zeros(4), np.zeros(4)
+    lib(variant).oracle_kerr_derivs(M, a, _dp(x), _dp(p), _dp(dx), _dp(dp))
+    return dx, dp
+
+
+def g_lower(M, a, x):
+    x = np.ascontiguousarray(x, np.float64)
+    g = np.zeros(16)
+    lib().oracle_g_lower(M, a, _dp(x), _dp(g))
+    return g.reshape(4, 4)
+
+
+def hamiltonian(M, a, x, p):
+    x = np.ascontiguousarray(x, np.float64)
+    p = np.ascontiguousarray(p, np.float64)
+    return lib().oracle_hamiltonian(M, a, _dp(x), _dp(p))
+
+
+def horizon(M, a):
+    return lib().oracle_horizon(M, a)
+
+
+def isco(M, a):
+    return lib().oracle_isco(M, a)
+
+
+def dp45_attempt(M, a, y, dl):
+    y = np.ascontiguousarray(y, np.float64)
+    out, err = np.zeros(8), C.c_double()
+    lib().oracle_dp45_attempt(M, a, _dp(y), dl, _dp(out), C.byref(err))
+    return out, err.value
+
+
+def rk4_step(M, a, y, dl):
+    y = np.ascontiguousarray(y, np.float64)
+    out = np.zeros(8)
+    lib().oracle_rk4_step(M, a, _dp(y), dl, _dp(out))
+    return out
+
+
+def adaptive_step(M, a, y, dl, tol):
+    y = np.ascontiguousarray(y, np.float64)
+    out, n = np.zeros(8), C.c_int()
+    nxt = lib().oracle_adaptive_step(M, a, _dp(y), dl, tol, _dp(out), C.byref(n))
+    return out, nxt, n.value
+
+
+def pixel_hash(i, j, s, ch):
+    return lib().oracle_pixel_hash(i, j, s, ch)
+
+
+def seeded_jitter(seed, idx):
+    jx, jy = C.c_double(), C.c_double()
+    lib().oracle_seeded_jitter(seed, idx, C.byref(jx), C.byref(jy))
+    return jx.value, jy.value
+
+
+def camera_ray(sc: OracleScene, px, py):
+    out = np.zeros(8)
+    lib().oracle_camera_ray(C.byref(sc), px, py, _dp(out))
+    return out
+
+
+def flux_lut(sc: OracleScene):
+    out, fmax = np.zeros(500), C.c_double()
+    lib().oracle_flux_lut(C.byref(sc), _dp(out), C.byref(fmax))
+    return out, fmax.value
+
+
+def spectrum_lut():
+    out = np.zeros(3000)
+    lib().oracle_spectrum_lut(_dp(out))
+    return out.reshape(1000, 3)
+
+
+def stars():
+    out = np.zeros(15000)
+    lib().oracle_stars(_dp(out))
+    return out.reshape(5000, 3)
+
+
+def star_sample(theta, phi):
+    out = np.zeros(3)
+    lib().oracle_star_sample(theta, phi, _dp(out))
+    return out
+
+
+def disk_emission(sc: OracleScene, rc, p_t, p_phi):
+    out, g, T = np.zeros(3), C.c_double(), C.c_double()
+    lib().oracle_disk_emission(C.byref(sc), rc, p_t, p_phi, _dp(out), C.byref(g), C.byref(T))
+    return out, g.value, T.value
+
+
+def trace(sc: OracleScene, state):
+    state = np.ascontiguousarray(state, np.float64)
+    r = OracleRay()
+    lib().oracle_trace(C.byref(sc), _dp(state), C.byref(r))
+    return r
+
+
+def trace_samples(sc: OracleScene, mode: int, spp: int, seed: int, ids, threads=0,
+                  variant="strict") -> np.ndarray:
+    ids = np.ascontiguousarray(ids, np.int64)
+    out = np.zeros(len(ids), RAY_DTYPE)
+    lib(variant).oracle_trace_samples(C.byref(sc), mode, spp, seed,
+                                      ids.ctypes.data_as(C.POINTER(C.c_int64)), len(ids),
+                                      out.ctypes.data, threads)
+    return out
+
+
+def render(sc: OracleScene, mode: int = JITTER_CENTER, spp: int = 1, seed: int = 0,
+           threads: int = 0, with_classes=True, variant="strict"):
+    """Full frame.  Returns (rgba[H,W,4] float32, classes[H*W*S] uint8 | None, steps)."""
+    if mode == JITTER_CENTER:
+        ns = 1
+    elif mode == JITTER_HASH:
+        g = max(1, int(np.sqrt(max(spp, 1))))
+        ns = g * g
+    else:
+        ns = max(spp, 1)
+    rgba = np.zeros((sc.height, sc.width, 4), np.float32)
+    cls = np.zeros(sc.width * sc.height * ns, np.uint8) if with_classes else None
+    steps = lib(variant).oracle_render(
+        C.byref(sc), mode, spp, seed, rgba.ctypes.data_as(C.POINTER(C.c_float)),
+        cls.ctypes.data_as(C.POINTER(C.c_uint8)) if cls is not None else None, threads)
+    return rgba, cls, steps

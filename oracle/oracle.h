@@ -1,0 +1,86 @@
+/*
+ * CPU oracle for the per-pixel null-geodesic + thin-disk path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain C++ restatement of the
+ * reference CPU algorithm used as the checker for the HIP path.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+ * it.  The product library (libgrrt.so) never links, loads or calls it.
+ *
+ * Each function cites the reference file:line it restates.  The reference
+ * itself is not buildable in this image (its headers include a CMake-
+ * generated export header), so every function here is pinned by closed
+ * forms, invariants and known-answer tests instead (see tests/test_oracle_*).
+ */
+#ifndef GRRT_ORACLE_H
+#define GRRT_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Scene description shared by every oracle entry point (geometric units). */
+typedef struct {
+    int width, height;
+    double mass;        /* M */
+    double spin_a;      /* dimensional a = a_* M */
+    double r_obs, theta_obs, phi_obs, fov;
+    int disk_enabled;
+    double r_in, r_out, t_peak;
+    int stars;          /* 1 = 5000-star background (seed 42) */
+    double tol;
+    int max_steps;
+    double r_escape;    /* reference: 1000 */
+} OracleScene;
+
+/* Per-ray record.  term: 0 horizon, 1 escaped, 2 max steps.
+ * cls: 0 captured, 1 escaped, 2 disk hit, 3 max steps (disk hit wins). */
+typedef struct {
+    double color[3];
+    double x_final[4];
+    double p_final[4];
+    double r_hit, g_hit;      /* first in-range disk crossing (0 if none) */
+    double h_max, h_final;    /* |H| max over accepted states, |H| at the end */
+    int32_t term, cls;
+    int32_t steps, attempts, n_cross, pad;
+} OracleRay;
+
+/* Sample placement modes for render/trace_rays. */
+enum { ORACLE_JITTER_HASH = 0, ORACLE_JITTER_CENTER = 1, ORACLE_JITTER_SEEDED = 2 };
+
+void   oracle_fast_sincos(double x, double* s, double* c);
+void   oracle_kerr_derivs(double M, double a, const double x[4], const double p[4],
+                          double dx[4], double dp[4]);
+void   oracle_g_lower(double M, double a, const double x[4], double g[16]);
+double oracle_hamiltonian(double M, double a, const double x[4], const double p[4]);
+double oracle_horizon(double M, double a);
+double oracle_isco(double M, double a);
+void   oracle_dp45_attempt(double M, double a, const double y[8], double dl,
+                           double y5[8], double* err);
+void   oracle_rk4_step(double M, double a, const double y[8], double dl, double out[8]);
+double oracle_adaptive_step(double M, double a, const double y[8], double dl, double tol,
+                            double out[8], int* attempts);
+double oracle_pixel_hash(int i, int j, int s, int ch);
+void   oracle_seeded_jitter(uint64_t seed, int64_t sample_index, double* jx, double* jy);
+void   oracle_camera_ray(const OracleScene* sc, double px, double py, double state[8]);
+void   oracle_flux_lut(const OracleScene* sc, double* lut500, double* flux_max);
+void   oracle_spectrum_lut(double* rgb3000);
+void   oracle_stars(double* theta_phi_b15000);
+void   oracle_star_sample(double theta, double phi, double out[3]);
+/* src/accretion_disk.cpp:132-147 for a crossing at rc with p_t, p_phi. */
+void   oracle_disk_emission(const OracleScene* sc, double rc, double p_t, double p_phi,
+                            double out[3], double* g, double* T);
+void   oracle_trace(const OracleScene* sc, const double state[8], OracleRay* out);
+/* Trace an explicit list of sample ids (pixel*spp + s); rays may be NULL. */
+void   oracle_trace_samples(const OracleScene* sc, int mode, int spp, uint64_t seed,
+                            const int64_t* ids, int64_t n, OracleRay* rays, int threads);
+/* Full frame: RGBA float W*H*4, per-sample classes (may be NULL). Returns
+ * the number of accepted steps summed over all rays. */
+int64_t oracle_render(const OracleScene* sc, int mode, int spp, uint64_t seed,
+                      float* rgba, uint8_t* cls, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

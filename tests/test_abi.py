@@ -1,0 +1,85 @@
+"""The C-ABI library loads, exports every symbol its headers declare, keeps
+the reference's GRRTParams layout, and fails loudly without a GPU.  No
+compute calls are made here."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", "grrt", "api.h"), os.path.join(ROOT, "include", "grrt_ext.h")]
+
+
+def declared_symbols():
+    names = []
+    for h in HEADERS:
+        text = open(h).read()
+        names += re.findall(r"GRRT_EXPORT[^;(]*?\b(grrt_\w+)\s*\(", text)
+    return names
+
+
+def test_headers_declare_reference_abi():
+    names = set(declared_symbols())
+    reference_16 = {"grrt_create", "grrt_destroy", "grrt_update_params", "grrt_render",
+                    "grrt_render_cb", "grrt_render_tile", "grrt_cancel", "grrt_progress",
+                    "grrt_error", "grrt_cuda_available", "grrt_last_error", "grrt_tonemap",
+                    "grrt_debug_pixel", "grrt_set_frequency_bins", "grrt_render_spectral",
+                    "grrt_render_spectral_cb", "grrt_render_spectral_to_fits_cb"}
+    assert reference_16 <= names
+
+
+def test_library_exports_every_declared_symbol():
+    import grrt_amd
+    L = grrt_amd.lib()
+    for name in declared_symbols():
+        assert hasattr(L, name), name
+    bound = {n for n, _, _ in grrt_amd.API}
+    assert set(declared_symbols()) == bound
+
+
+def test_params_layout_matches_reference():
+    import grrt_amd
+    P = grrt_amd.GRRTParams
+    assert C.sizeof(P) == 240
+    offs = {name: getattr(P, name).offset for name, _ in P._fields_}
+    expect = {"width": 0, "height": 4, "metric_type": 8, "mass": 16, "spin": 24, "observer_r": 32,
+              "observer_theta": 40, "observer_phi": 48, "fov": 56, "cam_yaw": 64, "cam_pitch": 72,
+              "cam_roll": 80, "disk_enabled": 88, "disk_inner": 96, "disk_outer": 104,
+              "disk_temperature": 112, "disk_volumetric": 120, "disk_alpha": 128,
+              "disk_turbulence": 136, "disk_seed": 144, "disk_noise_scale": 152,
+              "disk_noise_octaves": 160, "mass_solar": 168, "eddington_fraction": 176,
+              "background_type": 184, "background_texture_path": 192,
+              "integrator_tolerance": 200, "integrator_max_steps": 208,
+              "samples_per_pixel": 212, "thread_count": 216, "backend": 220,
+              "num_frequency_bins": 224, "frequency_bins_hz": 232}
+    assert offs == expect
+
+
+def test_no_gpu_fails_loudly():
+    import grrt_amd
+    if grrt_amd.gpu_available():
+        pytest.skip("a GPU is present")
+    with pytest.raises(RuntimeError, match="grrt_create failed"):
+        grrt_amd.Context(grrt_amd.make_params(width=8, height=8))
+
+
+def test_tonemap_host_only():
+    """grrt_tonemap is host post-processing (src/tonemapper.cpp:7-71):
+    log-average exposure, extended Reinhard, gamma 2.2."""
+    import numpy as np
+
+    import grrt_amd
+    rng = np.random.default_rng(0)
+    fb = np.zeros((8, 8, 4), np.float32)
+    fb[..., :3] = rng.lognormal(size=(8, 8, 3)).astype(np.float32)
+    fb[0, 0, :3] = 0.0
+    out = grrt_amd.tonemap(fb)
+    assert np.all(out[..., :3] >= 0) and np.all(out[..., :3] <= 1.0)
+    assert np.all(out[0, 0, :3] == 0)
+    lum = 0.2126 * fb[..., 0] + 0.7152 * fb[..., 1] + 0.0722 * fb[..., 2]
+    lum_o = 0.2126 * out[..., 0] + 0.7152 * out[..., 1] + 0.0722 * out[..., 2]
+    order = np.argsort(lum.ravel())[1:]
+    assert np.all(np.diff(lum_o.ravel()[order]) >= -0.05)   # brightness order kept
+    black = np.zeros((4, 4, 4), np.float32)
+    assert np.array_equal(grrt_amd.tonemap(black), black)

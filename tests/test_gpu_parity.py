@@ -1,0 +1,163 @@
+"""GPU parity of the HIP thin-disk tracer against the CPU oracle.
+
+Bars (north star): per-ray outcome class agrees on >= 99.9% of rays with
+mismatches only next to a class boundary (the critical curve or a disk
+edge); linear HDR median relative error <= 1e-6; Hamiltonian-constraint
+drift no worse than the oracle's.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from parity_util import class_boundary_mask, median_rel_err
+
+pytestmark = pytest.mark.gpu
+
+C1 = dict(width=256, height=256, metric=0, spin=0.0, observer_r=50.0,
+          observer_theta=math.radians(80.0), fov=math.radians(90.0), disk=True,
+          disk_outer=20.0, stars=False)
+
+
+def _oracle_scene(O, p, stars=None):
+    a = p.spin * p.mass if p.metric_type == 1 else 0.0
+    return O.scene(width=p.width, height=p.height, mass=p.mass, spin_a=a,
+                   r_obs=p.observer_r, theta_obs=p.observer_theta, phi_obs=p.observer_phi,
+                   fov=p.fov, disk=bool(p.disk_enabled),
+                   r_in=p.disk_inner if p.disk_inner > 0 else None,
+                   r_out=p.disk_outer, t_peak=p.disk_temperature or 1e7,
+                   stars=(p.background_type == 1) if stars is None else stars,
+                   tol=p.integrator_tolerance, max_steps=p.integrator_max_steps)
+
+
+def test_c1_frame_parity(grrt, oracle):
+    print("case: C1 Schwarzschild 256^2 pixel centres", flush=True)
+    p = grrt.make_params(**C1)
+    with grrt.Context(p) as ctx:
+        ctx.set_sampling(grrt.SAMPLING_CENTER)
+        ctx.set_records(True)
+        fb = ctx.render()
+        rec = ctx.records()
+        st = ctx.stats()
+    sc = _oracle_scene(oracle, p)
+    ofb, ocls, _ = oracle.render(sc, oracle.JITTER_CENTER, 1, threads=0)
+    ids = np.arange(p.width * p.height)
+    orec = oracle.trace_samples(sc, oracle.JITTER_CENTER, 1, 0, ids)
+    cls = rec["cls"].astype(np.uint8)
+    agree = np.mean(cls == ocls)
+    assert agree >= 0.999, agree
+    mism = (cls != ocls).reshape(p.height, p.width)
+    edge = class_boundary_mask(ocls.reshape(p.height, p.width))
+    assert not np.any(mism & ~edge), "class mismatch away from a class boundary"
+    same = (cls == ocls).reshape(p.height, p.width)
+    mre = median_rel_err(fb[..., :3][same], ofb[..., :3][same])
+    assert mre <= 1e-6, mre
+    assert np.all(fb[..., 3] == 1.0)
+    # Hamiltonian drift no worse than the oracle's (median and 99th pct).
+    g, o = rec["h_max"], orec["h_max"]
+    assert np.median(g) <= max(2.0 * np.median(o), 1e-13)
+    assert np.percentile(g, 99) <= max(2.0 * np.percentile(o, 99), 1e-12)
+    assert st.rays == p.width * p.height
+    assert st.steps == int(rec["steps"].sum())
+    assert st.attempts == int(rec["attempts"].sum())
+
+
+def test_shadow_closed_form(grrt):
+    """Disk off, a=0: captured iff the angle to the optical axis is below the
+    static-observer shadow angle sin(psi) = 3 sqrt(3) M sqrt(1-2M/r)/r."""
+    p = grrt.make_params(**{**C1, "disk": False})
+    with grrt.Context(p) as ctx:
+        ctx.set_sampling(grrt.SAMPLING_CENTER)
+        ctx.render()
+        cls = ctx.classes().reshape(p.height, p.width)
+    W = p.width
+    al = (np.arange(W) + 0.5 - W / 2) * p.fov / W
+    A, B = np.meshgrid(al, al)
+    ang = np.arccos(np.cos(A) * np.cos(B))
+    psi = math.asin(3 * math.sqrt(3) * math.sqrt(1 - 2 / 50.0) / 50.0)
+    pred = np.where(ang < psi, 0, 1)
+    assert np.array_equal(pred, cls)
+
+
+def test_tile_equals_frame(grrt):
+    p = grrt.make_params(width=96, height=64, spin=0.9, observer_theta=1.3, fov=1.2,
+                         stars=True, spp=4)
+    with grrt.Context(p) as ctx:
+        full = ctx.render()
+        tile = ctx.render_tile(17, 9, 40, 31)
+    assert np.array_equal(tile, full[9:40, 17:57])
+
+
+def test_hash_sampling_abi_parity(grrt, oracle):
+    """grrt_render's own stratified jitter (src/renderer.cpp:14-55), Kerr,
+    stars on, 4 spp."""
+    print("case: Kerr a=0.998 64x48 hash 4spp stars", flush=True)
+    p = grrt.make_params(width=64, height=48, spin=0.998, observer_theta=1.3962634,
+                         fov=math.radians(60), stars=True, spp=4)
+    with grrt.Context(p) as ctx:
+        ctx.set_records(True)
+        fb = ctx.render()
+        rec = ctx.records()
+    sc = _oracle_scene(oracle, p)
+    ofb, ocls, _ = oracle.render(sc, oracle.JITTER_HASH, 4)
+    assert np.mean(rec["cls"] == ocls) >= 0.995
+    px_same = np.all((rec["cls"] == ocls).reshape(-1, 4), axis=1).reshape(48, 64)
+    mre = median_rel_err(fb[..., :3][px_same], ofb[..., :3][px_same])
+    assert mre <= 1e-6, mre
+
+
+def test_c2_sampled_parity(grrt, oracle):
+    """The 1024^2 4-spp Kerr a=0.998 frame with seeded jitter (seed 1234),
+    compared ray by ray with the oracle on 4096 sampled rays."""
+    print("case: C2 Kerr a=0.998 1024^2 4spp seeded 1234 stars", flush=True)
+    p = grrt.make_params(width=1024, height=1024, spin=0.998, observer_theta=math.radians(80),
+                         fov=math.radians(90), stars=True, spp=4)
+    with grrt.Context(p) as ctx:
+        ctx.set_sampling(grrt.SAMPLING_SEEDED, 4, 1234)
+        ctx.set_records(True)
+        fb = ctx.render()
+        rec = ctx.records()
+    assert np.isfinite(fb).all()
+    rng = np.random.default_rng(5)
+    ids = np.sort(rng.choice(len(rec), 4096, replace=False))
+    sc = _oracle_scene(oracle, p)
+    orec = oracle.trace_samples(sc, oracle.JITTER_SEEDED, 4, 1234, ids)
+    g = rec[ids]
+    agree = np.mean(g["cls"] == orec["cls"])
+    assert agree >= 0.999, agree
+    same = g["cls"] == orec["cls"]
+    mre = median_rel_err(g["color"][same], orec["color"][same])
+    assert mre <= 1e-6, mre
+    # size-independent properties of the full frame
+    counts = np.bincount(rec["cls"], minlength=4)
+    assert counts[3] == 0 and counts[0] > 0 and counts[2] > 0
+    assert np.all(rec["steps"] > 0)
+
+
+def test_edge_cases(grrt, oracle):
+    """1x1 frame, max_steps exhaustion, a tiny tolerance (dl_min path)."""
+    p = grrt.make_params(width=1, height=1, spin=0.5, fov=0.01)
+    with grrt.Context(p) as ctx:
+        fb = ctx.render()
+    assert fb.shape == (1, 1, 4) and np.isfinite(fb).all()
+    p = grrt.make_params(width=32, height=32, spin=0.9, max_steps=25)
+    with grrt.Context(p) as ctx:
+        ctx.set_sampling(grrt.SAMPLING_CENTER)
+        ctx.set_records(True)
+        ctx.render()
+        rec = ctx.records()
+    sc = _oracle_scene(oracle, p)
+    orec = oracle.trace_samples(sc, oracle.JITTER_CENTER, 1, 0, np.arange(32 * 32))
+    assert np.mean(rec["cls"] == orec["cls"]) >= 0.99
+    assert np.all(rec["steps"] <= 25)
+    assert np.any(rec["cls"] == 3)
+    p = grrt.make_params(width=16, height=16, spin=0.7, tolerance=1e-15, max_steps=200)
+    with grrt.Context(p) as ctx:
+        ctx.set_sampling(grrt.SAMPLING_CENTER)
+        ctx.set_records(True)
+        ctx.render()
+        rec = ctx.records()
+    sc = _oracle_scene(oracle, p)
+    orec = oracle.trace_samples(sc, oracle.JITTER_CENTER, 1, 0, np.arange(256))
+    assert np.mean(rec["cls"] == orec["cls"]) >= 0.98
+    assert np.array_equal(rec["steps"], orec["steps"]) or np.mean(rec["steps"] == orec["steps"]) > 0.9
