@@ -77,9 +77,15 @@ def test_tonemap_host_only():
     out = grrt_amd.tonemap(fb)
     assert np.all(out[..., :3] >= 0) and np.all(out[..., :3] <= 1.0)
     assert np.all(out[0, 0, :3] == 0)
+    # chrominance is kept: undoing gamma gives one scale factor per pixel
+    lin = out[..., :3].astype(np.float64) ** 2.2
+    ok = np.all(lin < 0.999, axis=-1) & np.all(fb[..., :3] > 0, axis=-1)
+    ratio = lin[ok] / fb[..., :3][ok]
+    np.testing.assert_allclose(ratio, ratio[:, :1].repeat(3, 1), rtol=1e-4)
+    # the mapped luminance is monotone in the input luminance
     lum = 0.2126 * fb[..., 0] + 0.7152 * fb[..., 1] + 0.0722 * fb[..., 2]
-    lum_o = 0.2126 * out[..., 0] + 0.7152 * out[..., 1] + 0.0722 * out[..., 2]
-    order = np.argsort(lum.ravel())[1:]
-    assert np.all(np.diff(lum_o.ravel()[order]) >= -0.05)   # brightness order kept
+    lum_o = (lin * np.array([0.2126, 0.7152, 0.0722])).sum(-1)
+    order = np.argsort(lum[ok])
+    assert np.all(np.diff(lum_o[ok][order]) >= -1e-6)
     black = np.zeros((4, 4, 4), np.float32)
     assert np.array_equal(grrt_amd.tonemap(black), black)
