@@ -108,7 +108,7 @@ API = [
     ("grrt_ext_get_classes", C.c_int, [_P, _P, C.c_int64]),
     ("grrt_ext_last_stats", C.c_int, [_P, C.POINTER(GRRTStats)]),
     ("grrt_ext_render_device", C.c_int, [_P, _P]),
-    ("grrt_ext_set_tuning", C.c_int, [_P, C.c_int, C.c_int, C.c_int]),
+    ("grrt_ext_set_tuning", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
 ]
 
 _lib: C.CDLL | None = None
@@ -203,8 +203,9 @@ class Context:
     def set_records(self, enable: bool = True):
         self._check(self._L.grrt_ext_set_records(self._h, 1 if enable else 0), "set_records")
 
-    def set_tuning(self, block=0, blocks_per_cu=0, refill_min=0):
-        self._check(self._L.grrt_ext_set_tuning(self._h, block, blocks_per_cu, refill_min), "set_tuning")
+    def set_tuning(self, block=0, blocks_per_cu=0, refill_min=0, chunk=0):
+        self._check(self._L.grrt_ext_set_tuning(self._h, block, blocks_per_cu, refill_min, chunk),
+                    "set_tuning")
 
     def render(self, progress=None) -> np.ndarray:
         fb = np.zeros((self.params.height, self.params.width, 4), np.float32)

@@ -100,6 +100,7 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, float* d_out
     check(ctx->sample_cls.ensure(static_cast<size_t>(n_rays)), "hipMalloc(classes)");
     if (ctx->keep_records) check(ctx->records.ensure(static_cast<size_t>(n_rays)), "hipMalloc(records)");
     check(ctx->counters.ensure(1), "hipMalloc(counters)");
+    check(ctx->params_dev.ensure(1), "hipMalloc(params)");
 
     TraceParams P{};
     P.M = r.M;
@@ -152,14 +153,20 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, float* d_out
     P.counters = ctx->counters.ptr;
     P.cancel = ctx->cancel_flag_dev;
     P.refill_min = std::clamp(ctx->tune_refill, 1, 64);
+    P.chunk = std::max(ctx->tune_chunk, 64);
 
     *ctx->cancel_flag = 0;
     ctx->progress.store(0.0f);
     const int threads = ctx->tune_block;
     const int blocks = std::max(1, ctx->num_cus * ctx->tune_blocks_per_cu);
+    // The stream is idle here (every render ends with a synchronise), so the
+    // pinned staging copy can be rewritten.
+    *ctx->params_host = P;
+    check(hipMemcpyAsync(ctx->params_dev.ptr, ctx->params_host, sizeof(TraceParams), hipMemcpyHostToDevice,
+                         ctx->stream), "upload params");
     check(launch_reset(P.counters, ctx->stream), "reset kernel");
     check(hipEventRecord(ctx->ev[0], ctx->stream), "hipEventRecord");
-    check(launch_trace_thin(P, blocks, threads, ctx->stream), "trace kernel launch");
+    check(launch_trace_thin(ctx->params_dev.ptr, blocks, threads, ctx->stream), "trace kernel launch");
     check(hipEventRecord(ctx->ev[1], ctx->stream), "hipEventRecord");
     check(launch_finalize(P.sample_rgb, d_out, npix, nS, P.counters, ctx->stream), "finalize launch");
     check(hipEventRecord(ctx->ev[2], ctx->stream), "hipEventRecord");
@@ -271,6 +278,9 @@ GRRTContext* grrt_create(const GRRTParams* params) {
         void* dflag = nullptr;
         check(hipHostGetDevicePointer(&dflag, flag, 0), "hipHostGetDevicePointer");
         ctx->cancel_flag_dev = static_cast<const volatile int*>(dflag);
+        void* staging = nullptr;
+        check(hipHostMalloc(&staging, sizeof(grrt_amd::TraceParams), hipHostMallocDefault), "hipHostMalloc");
+        ctx->params_host = static_cast<grrt_amd::TraceParams*>(staging);
         build_scene(ctx);
         return ctx;
     } catch (const std::exception& e) {
@@ -294,6 +304,8 @@ void grrt_destroy(GRRTContext* ctx) {
     ctx->records.release();
     ctx->frame.release();
     ctx->counters.release();
+    ctx->params_dev.release();
+    if (ctx->params_host) (void)hipHostFree(ctx->params_host);
     if (ctx->cancel_flag) (void)hipHostFree(const_cast<int*>(ctx->cancel_flag));
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -502,7 +514,7 @@ int grrt_ext_render_device(GRRTContext* ctx, float* d_rgba) {
     });
 }
 
-int grrt_ext_set_tuning(GRRTContext* ctx, int block, int blocks_per_cu, int refill_min) {
+int grrt_ext_set_tuning(GRRTContext* ctx, int block, int blocks_per_cu, int refill_min, int chunk) {
     return guarded(ctx, [&] {
         if (block > 0) {
             if (block % 64 != 0 || block > 256) throw HipFailure("block must be a multiple of 64, at most 256");
@@ -510,6 +522,10 @@ int grrt_ext_set_tuning(GRRTContext* ctx, int block, int blocks_per_cu, int refi
         }
         if (blocks_per_cu > 0) ctx->tune_blocks_per_cu = blocks_per_cu;
         if (refill_min > 0) ctx->tune_refill = std::min(refill_min, 64);
+        if (chunk > 0) {
+            if (chunk < 64) throw HipFailure("chunk must be at least 64 rays");
+            ctx->tune_chunk = chunk;
+        }
         return 0;
     });
 }

@@ -36,7 +36,7 @@ struct DevBuf {
 };
 
 hipError_t launch_reset(DevCounters* c, hipStream_t st);
-hipError_t launch_trace_thin(const TraceParams& P, int blocks, int threads, hipStream_t st);
+hipError_t launch_trace_thin(const TraceParams* P, int blocks, int threads, hipStream_t st);
 hipError_t launch_finalize(const double* rgb, float* out, long long npix, int nS,
                            const DevCounters* cnt, hipStream_t st);
 double flops_per_attempt();
@@ -66,6 +66,8 @@ struct GRRTContext {
     grrt_amd::DevBuf<GRRTRayRecord> records;
     grrt_amd::DevBuf<float> frame;
     grrt_amd::DevBuf<grrt_amd::DevCounters> counters;
+    grrt_amd::DevBuf<grrt_amd::TraceParams> params_dev;   // launch parameters
+    grrt_amd::TraceParams* params_host = nullptr;          // pinned staging copy
     volatile int* cancel_flag = nullptr;     // host-mapped, polled by the kernel
     const volatile int* cancel_flag_dev = nullptr;   // device alias of cancel_flag
 
@@ -77,7 +79,8 @@ struct GRRTContext {
     bool records_valid = false;
     int tune_block = 256;
     int tune_blocks_per_cu = 2;
-    int tune_refill = 16;
+    int tune_refill = 4;
+    int tune_chunk = 128;
 
     GRRTStats last{};
     long long last_rays = 0;

@@ -19,6 +19,13 @@ __device__ __forceinline__ double rcp(double x) {
     return __builtin_fma(r, e, r);
 }
 
+// 1/x with one Newton–Raphson step (about 1e-15 relative), for ratios that
+// only feed comparisons and the step controller.
+__device__ __forceinline__ double rcp1(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return __builtin_fma(r, __builtin_fma(-x, r, 1.0), r);
+}
+
 // Comparison semantics of std::min / std::max / std::clamp: a NaN in the
 // second operand of min/max is ignored, clamp passes a NaN through.
 __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }

@@ -65,6 +65,7 @@ struct TraceParams {
     const volatile int* cancel;         // host-mapped flag, may be nullptr
 
     int refill_min;                     // idle lanes that trigger a refill
+    int chunk;                          // rays claimed per queue atomic (>= 64)
 };
 
 }  // namespace grrt_amd

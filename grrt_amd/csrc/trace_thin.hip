@@ -260,15 +260,28 @@ __device__ __forceinline__ void finish_ray(const TraceParams& P, long long id, R
     }
 }
 
+// Loop-invariant scalars of the hot loop, read once per kernel.
+struct Hot {
+    KerrK K;
+    double r_term, r_escape, tol;
+    int max_steps, refill_min, disk, chunk;
+    long long n_rays;
+};
+
 // Termination test of src/geodesic_tracer.cpp:52-63 on the current state;
 // -1 while the ray continues.
-__device__ __forceinline__ int termination(const TraceParams& P, const Ray& R) {
-    if (R.steps >= P.max_steps) return 2;
+__device__ __forceinline__ int termination(const Hot& Q, const Ray& R) {
+    if (R.steps >= Q.max_steps) return 2;
     const double r = R.y.v[0];
-    if (r < P.r_term) return 0;
-    if (r > P.r_escape) return 1;
+    if (r < Q.r_term) return 0;
+    if (r > Q.r_escape) return 1;
     return -1;
 }
+
+// Compiler-only fence: keeps the parameter-block loads of the rare paths
+// (ray start, disk crossing, ray end) where they are used instead of
+// hoisting them out of the loop into registers.
+__device__ __forceinline__ void keep_loads_here() { asm volatile("" ::: "memory"); }
 
 __device__ __forceinline__ unsigned lane_id() {
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
@@ -283,54 +296,98 @@ __global__ void reset_counters(DevCounters* c) {
         reinterpret_cast<unsigned long long*>(c)[threadIdx.x] = 0ull;
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void trace_thin(TraceParams P) {
+// The parameter block lives in device memory; the hot scalars are read once
+// at entry, everything else where it is used.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void trace_thin(const TraceParams* Pg) {
     __shared__ double s_flux[kFluxN];
     __shared__ double s_spec[3 * kSpecN];
-    if (P.disk) {
-        for (int i = threadIdx.x; i < kFluxN; i += blockDim.x) s_flux[i] = P.flux_lut[i];
-        for (int i = threadIdx.x; i < 3 * kSpecN; i += blockDim.x) s_spec[i] = P.spec_lut[i];
+    const Hot Q{{Pg->M, Pg->a, Pg->a * Pg->a, 2.0 * Pg->M}, Pg->r_term, Pg->r_escape, Pg->tol,
+                Pg->max_steps, Pg->refill_min, Pg->disk, Pg->chunk, Pg->n_rays};
+    DevCounters* const counters = Pg->counters;
+    if (Q.disk) {
+        const double* fl = Pg->flux_lut;
+        const double* sl = Pg->spec_lut;
+        for (int i = threadIdx.x; i < kFluxN; i += blockDim.x) s_flux[i] = fl[i];
+        for (int i = threadIdx.x; i < 3 * kSpecN; i += blockDim.x) s_spec[i] = sl[i];
     }
     __syncthreads();
 
-    const KerrK K{P.M, P.a, P.a * P.a, 2.0 * P.M};
+    const KerrK& K = Q.K;
     const unsigned lane = lane_id();
     Ray R;
     long long id = -1;
     bool live = false;
-    bool queue_open = true;
     unsigned long long n_steps = 0, n_att = 0, n_rays = 0;
-    unsigned refills = 0;
+
+    // Ray indices are claimed from the device-wide queue in chunks of CH
+    // (>= 64) by lane 0.  The claim for the next chunk is issued as soon as
+    // the previous one is taken, so the atomic's latency overlaps with the
+    // integration; the chunk is read only when the current range runs dry.
+    // A cancel request stops further claims; claimed rays always complete,
+    // so the finished rays are exactly [0, queue_head).
+    const unsigned long long nR = static_cast<unsigned long long>(Q.n_rays);
+    const unsigned long long CH = static_cast<unsigned long long>(Q.chunk);
+    const volatile int* cancel = Pg->cancel;
+    unsigned long long cur = 0, cur_end = 0;     // wave-uniform claimed range
+    unsigned long long pre = 0;                  // lane 0: base of the claim in flight
+    int cancel_seen = 0;                         // lane 0: cancel flag read with that claim
+    bool have_pre = true;
+    bool queue_open = true;
+    if (lane == 0) {
+        pre = atomicAdd(&counters->queue_head, CH);
+        cancel_seen = cancel ? *cancel : 0;
+    }
 
     while (true) {
-        // ---- refill idle lanes from the ray queue (wave-uniform branch) ----
+        // ---- refill idle lanes (wave-uniform branch) ----
         const unsigned long long idle = __ballot(!live);
         const int n_idle = __popcll(idle);
-        if (queue_open && (n_idle >= P.refill_min || n_idle == 64)) {
-            const int leader = __ffsll(static_cast<long long>(idle)) - 1;
-            unsigned long long base = 0;
-            if (lane == static_cast<unsigned>(leader)) {
-                // A cancelled render claims nothing more; every claimed ray completes.
-                const bool stop = P.cancel && (refills & 7u) == 0u && *P.cancel != 0;
-                base = stop ? (~0ull >> 1)
-                            : atomicAdd(&P.counters->queue_head, static_cast<unsigned long long>(n_idle));
+        if (queue_open && (n_idle >= Q.refill_min || n_idle == 64)) {
+            const unsigned long long avail = cur_end - cur;
+            unsigned long long nb = ~0ull;
+            if (avail < static_cast<unsigned long long>(n_idle) && have_pre) {
+                const unsigned long long got = __shfl(pre, 0);
+                const int stop = __shfl(cancel_seen, 0);
+                have_pre = false;
+                if (got < nR) {
+                    nb = got;
+                    if (!stop) {
+                        if (lane == 0) {
+                            pre = atomicAdd(&counters->queue_head, CH);
+                            cancel_seen = cancel ? *cancel : 0;
+                        }
+                        have_pre = true;
+                    }
+                }
             }
-            base = __shfl(base, leader);
-            ++refills;
-            if (base + n_idle >= static_cast<unsigned long long>(P.n_rays)) queue_open = false;
             if (!live) {
-                const unsigned long long mine = base + lanes_below(idle);
-                if (mine < static_cast<unsigned long long>(P.n_rays)) {
+                const unsigned long long k = lanes_below(idle);
+                unsigned long long mine = ~0ull;
+                if (k < avail) mine = cur + k;
+                else if (nb != ~0ull) mine = nb + (k - avail);
+                if (mine < nR) {
                     id = static_cast<long long>(mine);
-                    init_ray(P, K, id, R);
+                    keep_loads_here();
+                    init_ray(*Pg, K, id, R);
                     live = true;
-                    const int term = termination(P, R);
+                    const int term = termination(Q, R);
                     if (term >= 0) {
-                        finish_ray(P, id, R, term);
+                        finish_ray(*Pg, id, R, term);
                         live = false;
                         ++n_rays;
                     }
                 }
             }
+            if (avail >= static_cast<unsigned long long>(n_idle)) {
+                cur += n_idle;
+            } else if (nb != ~0ull) {
+                cur = nb + (n_idle - avail);
+                cur_end = nb + CH < nR ? nb + CH : nR;
+            } else {
+                cur = cur_end;
+            }
+            if (cur >= cur_end && !have_pre) queue_open = false;
         }
         if (__ballot(live) == 0ull) {
             if (!queue_open) break;
@@ -385,7 +442,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
             const double ei = __builtin_fma(k7.v[i], e7, e.v[i]);
-            err = smax(err, __builtin_fabs(ei) * rcp(__builtin_fabs(y5.v[i]) + 1e-10));
+            err = __builtin_fmax(err, __builtin_fabs(ei) * rcp1(__builtin_fabs(y5.v[i]) + 1e-10));
         }
         ++R.attempts;
 
@@ -399,13 +456,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
         if (R.retry >= 20) {
             accept = true;           // retries exhausted: take the last attempt
             next = h;
-        } else if (err <= P.tol) {
+        } else if (err <= Q.tol) {
             accept = true;
-            double f = (err > 1e-30) ? step_factor(P.tol / err) : 5.0;
+            double f = (err > 1e-30) ? step_factor(Q.tol / err) : 5.0;
             f = sclamp(f, 1.0, 5.0);
             next = smin(h * f, 5.0 * smax(__builtin_fabs(y5.v[0]), 1.0));
         } else {
-            double f = step_factor(P.tol / err);
+            double f = step_factor(Q.tol / err);
             f = sclamp(f, 0.2, 1.0);
             const double dl = h * f;
             if (dl < 1e-6) {
@@ -424,14 +481,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
         R.retry = 0;
 
         // ---- thin-disk crossing between the previous and new state ----
-        if (P.disk) {
+        if (Q.disk) {
             const double half_pi = kPi / 2.0;
             const double dp = R.y.v[1] - half_pi, dn = y5.v[1] - half_pi;
             if (dp * dn < 0.0 && __builtin_fabs(dp - dn) > 1e-12) {
                 const double frac = -dp / (dn - dp);
                 const double rc = R.y.v[0] + frac * (y5.v[0] - R.y.v[0]);
-                if (rc >= P.r_in && rc <= P.r_out)
-                    disk_emission(P, s_flux, s_spec, rc, R.P0, R.P3, R);
+                keep_loads_here();
+                if (rc >= Pg->r_in && rc <= Pg->r_out)
+                    disk_emission(*Pg, s_flux, s_spec, rc, R.P0, R.P3, R);
             }
         }
         R.y = y5;
@@ -439,12 +497,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
         ++R.steps;
         R.h_last = __builtin_fabs(hamiltonian_from_rhs(R, dt7));
         R.h_max = smax(R.h_max, R.h_last);
-        const int term = termination(P, R);
+        const int term = termination(Q, R);
         if (term >= 0) {
             n_steps += R.steps;
             n_att += R.attempts;
             ++n_rays;
-            finish_ray(P, id, R, term);
+            keep_loads_here();
+            finish_ray(*Pg, id, R, term);
             live = false;
         }
     }
@@ -456,9 +515,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void t
         n_rays += __shfl_xor(n_rays, off);
     }
     if (lane == 0) {
-        atomicAdd(&P.counters->steps, n_steps);
-        atomicAdd(&P.counters->attempts, n_att);
-        atomicAdd(&P.counters->rays_done, n_rays);
+        atomicAdd(&counters->steps, n_steps);
+        atomicAdd(&counters->attempts, n_att);
+        atomicAdd(&counters->rays_done, n_rays);
     }
 }
 
@@ -499,7 +558,7 @@ hipError_t launch_reset(DevCounters* c, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t launch_trace_thin(const TraceParams& P, int blocks, int threads, hipStream_t st) {
+hipError_t launch_trace_thin(const TraceParams* P, int blocks, int threads, hipStream_t st) {
     hipLaunchKernelGGL(trace_thin, dim3(blocks), dim3(threads), 0, st, P);
     return hipGetLastError();
 }

@@ -68,9 +68,11 @@ GRRT_EXPORT int grrt_ext_last_stats(const GRRTContext* ctx, GRRTStats* out);
  * or NULL to keep it in the context).  No host copy is made. */
 GRRT_EXPORT int grrt_ext_render_device(GRRTContext* ctx, float* d_rgba);
 /* Persistent-kernel launch shape: threads per block (multiple of 64),
- * blocks per CU, and the number of idle lanes (1..64) that triggers a
- * wavefront refill from the ray queue.  0 keeps the current value. */
-GRRT_EXPORT int grrt_ext_set_tuning(GRRTContext* ctx, int block, int blocks_per_cu, int refill_min);
+ * blocks per CU, the number of idle lanes (1..64) that triggers a
+ * wavefront refill, and the rays a wavefront claims from the device queue
+ * per atomic (>= 64).  0 keeps the current value. */
+GRRT_EXPORT int grrt_ext_set_tuning(GRRTContext* ctx, int block, int blocks_per_cu, int refill_min,
+                                    int chunk);
 
 #ifdef __cplusplus
 }
