@@ -109,7 +109,21 @@ API = [
     ("grrt_ext_last_stats", C.c_int, [_P, C.POINTER(GRRTStats)]),
     ("grrt_ext_render_device", C.c_int, [_P, _P]),
     ("grrt_ext_set_tuning", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
+    ("grrt_ext_set_precision", C.c_int, [_P, C.c_int]),
+    ("grrt_ext_render_spectral_f32", C.c_int, [_P, C.POINTER(C.c_float), C.c_int, C.c_int]),
+    ("grrt_ext_vol_build", _P, [C.POINTER(GRRTParams)]),
+    ("grrt_ext_vol_scalars", C.c_int, [_P, C.POINTER(C.c_double), C.c_int]),
+    ("grrt_ext_vol_table", C.c_int64, [_P, C.c_char_p, C.POINTER(C.c_double), C.c_int64]),
+    ("grrt_ext_vol_free", None, [_P]),
+    ("grrt_ext_saha", C.c_int, [C.c_double, C.c_double, C.POINTER(C.c_double)]),
+    ("grrt_ext_opacity", C.c_int, [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]),
 ]
+
+VOL_SCALARS = ["r_isco", "r_horizon", "r_min", "r_outer", "taper_width", "E_isco", "L_isco",
+               "rho_scale", "noise_scale", "turbulence", "octaves", "n_r", "n_z", "n_nu",
+               "log_nu_min", "log_nu_max", "M", "a", "t_peak"]
+VOL_TABLES = ["H", "z_max", "rho_mid", "T_eff", "rho_profile", "T_profile", "kappa_abs",
+              "kappa_es", "kappa_ross", "mu", "perm"]
 
 _lib: C.CDLL | None = None
 
@@ -142,7 +156,8 @@ def make_params(width=256, height=256, metric=METRIC_KERR, spin=0.998, mass=1.0,
                 observer_r=50.0, observer_theta=1.3962634, observer_phi=0.0,
                 fov=1.5707963, disk=True, disk_inner=0.0, disk_outer=20.0,
                 disk_temperature=0.0, stars=False, tolerance=1e-8, max_steps=10000,
-                spp=1, volumetric=False) -> GRRTParams:
+                spp=1, volumetric=False, alpha=0.1, turbulence=0.4, seed=42,
+                noise_scale=0.0, octaves=2, freqs=None) -> GRRTParams:
     p = GRRTParams()
     p.width, p.height = width, height
     p.metric_type = metric
@@ -155,7 +170,49 @@ def make_params(width=256, height=256, metric=METRIC_KERR, spin=0.998, mass=1.0,
     p.integrator_tolerance, p.integrator_max_steps = tolerance, max_steps
     p.samples_per_pixel = spp
     p.backend = BACKEND_GPU
+    p.disk_alpha, p.disk_turbulence, p.disk_seed = alpha, turbulence, seed
+    p.disk_noise_scale, p.disk_noise_octaves = noise_scale, octaves
+    if freqs is not None:
+        arr = (C.c_double * len(freqs))(*freqs)
+        p._freqs_keepalive = arr
+        p.num_frequency_bins = len(freqs)
+        p.frequency_bins_hz = C.cast(arr, C.POINTER(C.c_double))
     return p
+
+
+def volumetric_tables(params: GRRTParams) -> dict:
+    """Host-side volumetric disk set-up (no GPU): scalars and tables."""
+    L = lib()
+    h = L.grrt_ext_vol_build(C.byref(params))
+    if not h:
+        err = L.grrt_last_error()
+        raise RuntimeError("grrt_ext_vol_build failed: " + (err.decode() if err else "unknown"))
+    try:
+        sc = np.zeros(len(VOL_SCALARS))
+        L.grrt_ext_vol_scalars(h, sc.ctypes.data_as(C.POINTER(C.c_double)), len(sc))
+        out = dict(zip(VOL_SCALARS, sc.tolist()))
+        for name in VOL_TABLES:
+            n = L.grrt_ext_vol_table(h, name.encode(), None, 0)
+            arr = np.zeros(n)
+            L.grrt_ext_vol_table(h, name.encode(), arr.ctypes.data_as(C.POINTER(C.c_double)), n)
+            out[name] = arr
+        out["perm"] = out["perm"].astype(np.int32)
+        return out
+    finally:
+        L.grrt_ext_vol_free(h)
+
+
+def saha(rho, T) -> dict:
+    out = np.zeros(5)
+    lib().grrt_ext_saha(rho, T, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return dict(zip(["n_e", "mu", "n_ion_eff", "n_HI", "n_Hminus"], out.tolist()))
+
+
+def opacity(nu, rho, T) -> dict:
+    out = np.zeros(6)
+    lib().grrt_ext_opacity(nu, rho, T, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return dict(zip(["kappa_abs", "kappa_es", "alpha_ff", "alpha_hminus", "alpha_bf", "B_nu"],
+                    out.tolist()))
 
 
 class Context:
@@ -264,6 +321,18 @@ class Context:
         arr = np.ascontiguousarray(freqs, np.float64)
         self._freqs = arr
         self._L.grrt_set_frequency_bins(self._h, arr.ctypes.data_as(C.POINTER(C.c_double)), len(arr))
+
+    def set_precision(self, mixed: bool):
+        self._check(self._L.grrt_ext_set_precision(self._h, 1 if mixed else 0), "set_precision")
+
+    def render_spectral_f32(self, width=None, height=None) -> np.ndarray:
+        w = self.params.width if width is None else width
+        h = self.params.height if height is None else height
+        n = len(getattr(self, "_freqs", []))
+        buf = np.zeros((h, w, max(n, 1)), np.float32)
+        self._check(self._L.grrt_ext_render_spectral_f32(
+            self._h, buf.ctypes.data_as(C.POINTER(C.c_float)), w, h), "render_spectral_f32")
+        return buf
 
     def render_spectral(self, width=None, height=None) -> np.ndarray:
         w = self.params.width if width is None else width

@@ -42,11 +42,90 @@ void check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw HipFailure(std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Set-up time upload of a host table (synchronous: the source may be a temporary).
 template <class T>
 void upload(grrt_amd::DevBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
+    (void)st;
     check(b.ensure(v.size()), "hipMalloc(table)");
-    check(hipMemcpyAsync(b.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st),
-          "hipMemcpyAsync(table)");
+    check(hipMemcpy(b.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy(table)");
+}
+
+// Volumetric disk parameters exactly as the reference wires them
+// (src/api.cpp:79-127): the disk takes the raw mass and spin fields, its
+// own outer radius default (30 M), a peak temperature that may be derived
+// from the black-hole mass and Eddington fraction, and an opacity frequency
+// range widened to [0.1 nu_min, 10 nu_max] of the create-time bins.
+struct VolSetup {
+    double M, a, r_outer, t_peak;
+    grrt_amd::VolParams vp;
+};
+
+VolSetup resolve_volumetric(const GRRTParams& p) {
+    VolSetup v{};
+    double T = p.disk_temperature > 0 ? p.disk_temperature : 1e7;
+    if (p.mass_solar > 0.0 && p.eddington_fraction > 0.0 && p.disk_temperature <= 0.0) {
+        // radiative efficiency 1 - E_isco (Bardeen, Press & Teukolsky 1972)
+        const double as = std::abs(p.spin);
+        const double z1 = 1.0 + std::cbrt(1.0 - as * as) * (std::cbrt(1.0 + as) + std::cbrt(1.0 - as));
+        const double z2 = std::sqrt(3.0 * as * as + z1 * z1);
+        const double risco = 3.0 + z2 - std::sqrt((3.0 - z1) * (3.0 + z1 + 2.0 * z2));
+        const double vi = 1.0 / std::sqrt(risco);
+        const double E = (1.0 - 2.0 * vi * vi + as * vi * vi * vi) / std::sqrt(1.0 - 3.0 * vi * vi + 2.0 * as * vi * vi * vi);
+        const double eta = 1.0 - E;
+        T = 5.0e7 * std::pow(eta, 0.25) * std::pow(p.mass_solar, -0.25) * std::pow(p.eddington_fraction, 0.25);
+    }
+    v.M = p.mass;
+    v.a = p.spin;
+    v.r_outer = p.disk_outer > 0 ? p.disk_outer : 30.0;
+    v.t_peak = T;
+    v.vp.alpha = p.disk_alpha;
+    v.vp.turbulence = p.disk_turbulence;
+    v.vp.seed = static_cast<uint32_t>(p.disk_seed);
+    v.vp.noise_scale = p.disk_noise_scale;
+    v.vp.noise_octaves = p.disk_noise_octaves;
+    if (p.num_frequency_bins > 0 && p.frequency_bins_hz) {
+        double lo = p.frequency_bins_hz[0], hi = lo;
+        for (int i = 1; i < p.num_frequency_bins; ++i) {
+            lo = std::min(lo, p.frequency_bins_hz[i]);
+            hi = std::max(hi, p.frequency_bins_hz[i]);
+        }
+        v.vp.opacity_nu_min = lo * 0.1;
+        v.vp.opacity_nu_max = hi * 10.0;
+    }
+    return v;
+}
+
+void build_volumetric_tables(GRRTContext* ctx) {
+    using namespace grrt_amd;
+    const auto t0 = std::chrono::steady_clock::now();
+    const VolSetup vs = resolve_volumetric(ctx->params);
+    VolTables T;
+    build_volumetric(vs.M, vs.a, vs.r_outer, vs.t_peak, vs.vp, T);
+    std::vector<double> lnk(T.op.kappa_abs.size());
+    for (size_t i = 0; i < lnk.size(); ++i) lnk[i] = std::log(std::max(T.op.kappa_abs[i], 1e-100));
+    upload(ctx->vol_H, T.H, ctx->stream);
+    upload(ctx->vol_zmax, T.z_max, ctx->stream);
+    upload(ctx->vol_rho_mid, T.rho_mid, ctx->stream);
+    upload(ctx->vol_rho_prof, T.rho_profile, ctx->stream);
+    upload(ctx->vol_T_prof, T.T_profile, ctx->stream);
+    upload(ctx->vol_lnk, lnk, ctx->stream);
+    upload(ctx->vol_kes, T.op.kappa_es, ctx->stream);
+    upload(ctx->vol_perm, std::vector<int>(T.perm, T.perm + 512), ctx->stream);
+    check(hipStreamSynchronize(ctx->stream), "upload volumetric tables");
+    VolDev& V = ctx->vol;
+    V.M = T.M; V.a = T.a;
+    V.r_horizon = T.r_horizon; V.r_min = T.r_min; V.r_outer = T.r_outer; V.r_isco = T.r_isco;
+    V.taper_w = T.taper_width; V.E_isco = T.E_isco; V.L_isco = T.L_isco;
+    V.rho_scale = T.rho_scale; V.noise_scale = T.noise_scale; V.turbulence = T.prm.turbulence;
+    V.octaves = T.prm.noise_octaves; V.n_r = T.n_r; V.n_z = T.n_z;
+    V.H = ctx->vol_H.ptr; V.z_max = ctx->vol_zmax.ptr; V.rho_mid = ctx->vol_rho_mid.ptr;
+    V.rho_prof = ctx->vol_rho_prof.ptr; V.T_prof = ctx->vol_T_prof.ptr;
+    V.ln_kabs = ctx->vol_lnk.ptr; V.kes = ctx->vol_kes.ptr;
+    V.n_nu = T.op.n_nu; V.n_rho = T.op.n_rho; V.n_T = T.op.n_T;
+    V.lnu0 = T.op.log_nu_min; V.lnu1 = T.op.log_nu_max; V.lrho0 = T.op.log_rho_min;
+    V.lrho1 = T.op.log_rho_max; V.lT0 = T.op.log_T_min; V.lT1 = T.op.log_T_max;
+    V.perm = ctx->vol_perm.ptr;
+    ctx->vol_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 // (Re)build the scene tables from ctx->params.
@@ -69,6 +148,7 @@ void build_scene(GRRTContext* ctx) {
         upload(ctx->star_start, t.start, ctx->stream);
         upload(ctx->star_tpb, t.tpb, ctx->stream);
     }
+    if (r.volumetric) build_volumetric_tables(ctx);
     ctx->sampling_spp = r.spp;
     check(hipStreamSynchronize(ctx->stream), "upload tables");
 }
@@ -83,19 +163,23 @@ int samples_per_pixel(int mode, int spp, int& grid) {
     return std::max(spp, 1);
 }
 
-// Render the pixel window [x0,x0+tw) x [y0,y0+th) into device memory `out`
-// (tw*th*4 floats).  Returns 0, or 1 if cancelled.
-int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, float* d_out,
-                  grrt_progress_fn progress, void* ud) {
+enum class Output { kRGB, kSpectralF64, kSpectralF32 };
+
+// Render the pixel window [x0,x0+tw) x [y0,y0+th) into device memory `out`:
+// RGBA float32 (tw*th*4), or tw*th*nbins spectral values (double or float).
+// Returns 0, or 1 if cancelled.
+int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
+                  grrt_progress_fn progress, void* ud, Output kind = Output::kRGB) {
     using namespace grrt_amd;
     const auto t_start = std::chrono::steady_clock::now();
     const Resolved& r = ctx->res;
-    if (r.volumetric)
-        throw HipFailure("the volumetric disk is not available in this build of the GPU backend");
+    const bool spectral = kind != Output::kRGB;
+    const bool vol = r.volumetric != 0;
     int grid = 1;
     const int nS = samples_per_pixel(ctx->sampling_mode, ctx->sampling_spp, grid);
     const long long npix = static_cast<long long>(tw) * th;
     const long long n_rays = npix * nS;
+    const int nb = spectral ? static_cast<int>(ctx->frequency_bins.size()) : 3;
     check(ctx->sample_rgb.ensure(static_cast<size_t>(3 * n_rays)), "hipMalloc(samples)");
     check(ctx->sample_cls.ensure(static_cast<size_t>(n_rays)), "hipMalloc(classes)");
     if (ctx->keep_records) check(ctx->records.ensure(static_cast<size_t>(n_rays)), "hipMalloc(records)");
@@ -154,6 +238,23 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, float* d_out
     P.cancel = ctx->cancel_flag_dev;
     P.refill_min = std::clamp(ctx->tune_refill, 1, 64);
     P.chunk = std::max(ctx->tune_chunk, 64);
+    if (vol) {
+        P.vol = ctx->vol;
+        P.nbins = nb;
+        if (spectral) {
+            std::vector<double> lg(nb);
+            for (int k = 0; k < nb; ++k) lg[k] = std::log10(ctx->frequency_bins[k]);
+            upload(ctx->nu_obs, ctx->frequency_bins, ctx->stream);
+            upload(ctx->lognu_obs, lg, ctx->stream);
+            const size_t rt = ctx->mixed_precision ? sizeof(float) : sizeof(double);
+            check(ctx->rt_scratch.ensure(4 * static_cast<size_t>(nb) * n_rays * rt), "hipMalloc(rt scratch)");
+            check(ctx->sample_spec.ensure(static_cast<size_t>(nb) * n_rays), "hipMalloc(spectra)");
+            P.nu_obs = ctx->nu_obs.ptr;
+            P.lognu_obs = ctx->lognu_obs.ptr;
+            P.rt_scratch = ctx->rt_scratch.ptr;
+            P.sample_spec = ctx->sample_spec.ptr;
+        }
+    }
 
     *ctx->cancel_flag = 0;
     ctx->progress.store(0.0f);
@@ -166,9 +267,16 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, float* d_out
                          ctx->stream), "upload params");
     check(launch_reset(P.counters, ctx->stream), "reset kernel");
     check(hipEventRecord(ctx->ev[0], ctx->stream), "hipEventRecord");
-    check(launch_trace_thin(ctx->params_dev.ptr, blocks, threads, ctx->stream), "trace kernel launch");
+    if (vol) check(launch_trace_vol(ctx->params_dev.ptr, n_rays, spectral, ctx->mixed_precision, ctx->stream),
+                   "volumetric trace launch");
+    else check(launch_trace_thin(ctx->params_dev.ptr, blocks, threads, ctx->stream), "trace kernel launch");
     check(hipEventRecord(ctx->ev[1], ctx->stream), "hipEventRecord");
-    check(launch_finalize(P.sample_rgb, d_out, npix, nS, P.counters, ctx->stream), "finalize launch");
+    if (spectral)
+        check(launch_finalize_spectral(P.sample_spec, d_out, npix, nS, nb, kind == Output::kSpectralF32, ctx->stream),
+              "finalize launch");
+    else
+        check(launch_finalize(P.sample_rgb, static_cast<float*>(d_out), npix, nS, P.counters, ctx->stream),
+              "finalize launch");
     check(hipEventRecord(ctx->ev[2], ctx->stream), "hipEventRecord");
 
     // Wait on the caller's thread, reporting coarse progress from elapsed
@@ -201,6 +309,30 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, float* d_out
     ctx->progress.store(1.0f);
     if (progress) progress(1.0f, ud);
     return cancelled ? 1 : 0;
+}
+
+// Spectral render of a width x height window into host memory.
+int render_spectral_host(GRRTContext* ctx, void* host, int width, int height, bool f32,
+                         grrt_progress_fn progress, void* ud) {
+    if (ctx->frequency_bins.empty()) {
+        ctx->error_msg = "No frequency bins set -- call grrt_set_frequency_bins first";
+        return -1;
+    }
+    if (!host || width <= 0 || height <= 0) throw HipFailure("invalid spectral buffer");
+    const size_t n = static_cast<size_t>(width) * height * ctx->frequency_bins.size();
+    const size_t bytes = n * (f32 ? sizeof(float) : sizeof(double));
+    if (!ctx->res.volumetric) {
+        // Without a volumetric disk every spectrum is zero
+        // (src/geodesic_tracer.cpp:503-505).
+        std::memset(host, 0, bytes);
+        if (progress) progress(1.0f, ud);
+        return 0;
+    }
+    check(ctx->spec_out.ensure(bytes), "hipMalloc(spectral out)");
+    const int rc = render_window(ctx, 0, 0, width, height, ctx->spec_out.ptr, progress, ud,
+                                 f32 ? Output::kSpectralF32 : Output::kSpectralF64);
+    check(hipMemcpy(host, ctx->spec_out.ptr, bytes, hipMemcpyDeviceToHost), "copy spectra");
+    return rc;
 }
 
 void tonemap_host(float* fb, int w, int h) {
@@ -414,20 +546,7 @@ int grrt_render_spectral(GRRTContext* ctx, double* spectral_buffer, int width, i
 int grrt_render_spectral_cb(GRRTContext* ctx, double* spectral_buffer, int width, int height,
                             grrt_progress_fn progress, void* user_data) {
     return guarded(ctx, [&] {
-        if (ctx->frequency_bins.empty()) {
-            ctx->error_msg = "No frequency bins set -- call grrt_set_frequency_bins first";
-            return -1;
-        }
-        if (!spectral_buffer || width <= 0 || height <= 0) throw HipFailure("invalid spectral buffer");
-        if (!ctx->res.volumetric) {
-            // Without a volumetric disk every spectrum is zero
-            // (src/geodesic_tracer.cpp:503-505).
-            std::fill_n(spectral_buffer,
-                        static_cast<size_t>(width) * height * ctx->frequency_bins.size(), 0.0);
-            if (progress) progress(1.0f, user_data);
-            return 0;
-        }
-        throw HipFailure("the volumetric spectral path is not available in this build of the GPU backend");
+        return render_spectral_host(ctx, spectral_buffer, width, height, false, progress, user_data);
     });
 }
 
@@ -528,6 +647,103 @@ int grrt_ext_set_tuning(GRRTContext* ctx, int block, int blocks_per_cu, int refi
         }
         return 0;
     });
+}
+
+int grrt_ext_set_precision(GRRTContext* ctx, int mode) {
+    return guarded(ctx, [&] {
+        if (mode != 0 && mode != 1) throw HipFailure("precision mode must be 0 (FP64) or 1 (mixed)");
+        ctx->mixed_precision = mode == 1;
+        return 0;
+    });
+}
+
+int grrt_ext_render_spectral_f32(GRRTContext* ctx, float* out, int width, int height) {
+    return guarded(ctx, [&] { return render_spectral_host(ctx, out, width, height, true, nullptr, nullptr); });
+}
+
+}  // extern "C"
+
+struct GRRTVolTables {
+    grrt_amd::VolTables t;
+};
+
+extern "C" {
+
+GRRTVolTables* grrt_ext_vol_build(const GRRTParams* params) {
+    if (!params) { g_last_error = "grrt_ext_vol_build: params is null"; return nullptr; }
+    try {
+        const VolSetup vs = resolve_volumetric(*params);
+        auto* h = new GRRTVolTables();
+        grrt_amd::build_volumetric(vs.M, vs.a, vs.r_outer, vs.t_peak, vs.vp, h->t);
+        return h;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+    }
+    return nullptr;
+}
+
+int grrt_ext_vol_scalars(const GRRTVolTables* h, double* out, int cap) {
+    if (!h || !out) return -1;
+    const auto& t = h->t;
+    const double v[] = {t.r_isco, t.r_horizon, t.r_min, t.r_outer, t.taper_width, t.E_isco, t.L_isco,
+                        t.rho_scale, t.noise_scale, t.prm.turbulence, static_cast<double>(t.prm.noise_octaves),
+                        static_cast<double>(t.n_r), static_cast<double>(t.n_z), static_cast<double>(t.op.n_nu),
+                        t.op.log_nu_min, t.op.log_nu_max, t.M, t.a, t.t_peak};
+    const int n = static_cast<int>(sizeof(v) / sizeof(v[0]));
+    for (int i = 0; i < n && i < cap; ++i) out[i] = v[i];
+    return n;
+}
+
+int64_t grrt_ext_vol_table(const GRRTVolTables* h, const char* name, double* out, int64_t cap) {
+    if (!h || !name) return -1;
+    const auto& t = h->t;
+    const std::string n(name);
+    std::vector<double> perm;
+    const std::vector<double>* v = nullptr;
+    if (n == "H") v = &t.H;
+    else if (n == "z_max") v = &t.z_max;
+    else if (n == "rho_mid") v = &t.rho_mid;
+    else if (n == "T_eff") v = &t.T_eff;
+    else if (n == "rho_profile") v = &t.rho_profile;
+    else if (n == "T_profile") v = &t.T_profile;
+    else if (n == "kappa_abs") v = &t.op.kappa_abs;
+    else if (n == "kappa_es") v = &t.op.kappa_es;
+    else if (n == "kappa_ross") v = &t.op.kappa_ross;
+    else if (n == "mu") v = &t.op.mu;
+    else if (n == "perm") {
+        perm.assign(t.perm, t.perm + 512);
+        v = &perm;
+    } else {
+        return -1;
+    }
+    const int64_t len = static_cast<int64_t>(v->size());
+    if (out && cap >= len) std::memcpy(out, v->data(), v->size() * sizeof(double));
+    return len;
+}
+
+void grrt_ext_vol_free(GRRTVolTables* h) { delete h; }
+
+int grrt_ext_saha(double rho, double T, double out[5]) {
+    if (!out) return -1;
+    const grrt_amd::Ionization s = grrt_amd::saha(rho, T);
+    out[0] = s.n_e;
+    out[1] = s.mu;
+    out[2] = s.n_ion_eff;
+    out[3] = s.n_HI;
+    out[4] = s.n_Hminus;
+    return 0;
+}
+
+int grrt_ext_opacity(double nu, double rho, double T, double out[6]) {
+    if (!out) return -1;
+    const grrt_amd::Ionization s = grrt_amd::saha(rho, T);
+    out[0] = grrt_amd::kappa_abs_point(nu, rho, T, s);
+    out[1] = grrt_amd::kappa_es_point(rho, s);
+    out[2] = grrt_amd::alpha_ff_point(nu, T, s);
+    out[3] = grrt_amd::alpha_hminus_point(nu, T, s);
+    out[4] = grrt_amd::alpha_bf_point(nu, T, s);
+    out[5] = grrt_amd::planck_nu_host(nu, T);
+    return 0;
 }
 
 }  // extern "C"

@@ -10,6 +10,7 @@
 #include "grrt/types.h"
 #include "grrt_ext.h"
 #include "scene_host.hpp"
+#include "volumetric_host.hpp"
 #include "trace_params.hpp"
 
 namespace grrt_amd {
@@ -36,6 +37,9 @@ struct DevBuf {
 };
 
 hipError_t launch_reset(DevCounters* c, hipStream_t st);
+hipError_t launch_trace_vol(const TraceParams* P, long long n_rays, bool spectral, bool fp32_rt, hipStream_t st);
+hipError_t launch_finalize_spectral(const double* spec, void* out, long long npix, int nS, int nb, bool f32,
+                                    hipStream_t st);
 hipError_t launch_trace_thin(const TraceParams* P, int blocks, int threads, hipStream_t st);
 hipError_t launch_finalize(const double* rgb, float* out, long long npix, int nS,
                            const DevCounters* cnt, hipStream_t st);
@@ -59,6 +63,17 @@ struct GRRTContext {
     double flux_max = 0.0;
     grrt_amd::DevBuf<double> flux, spec, star_tpb;
     grrt_amd::DevBuf<int> star_start;
+
+    // Volumetric disk tables in device memory.
+    grrt_amd::VolDev vol{};
+    double vol_build_ms = 0.0;
+    grrt_amd::DevBuf<double> vol_H, vol_zmax, vol_rho_mid, vol_rho_prof, vol_T_prof, vol_lnk, vol_kes;
+    grrt_amd::DevBuf<int> vol_perm;
+    grrt_amd::DevBuf<double> nu_obs, lognu_obs;
+    grrt_amd::DevBuf<unsigned char> rt_scratch;
+    grrt_amd::DevBuf<double> sample_spec;
+    grrt_amd::DevBuf<unsigned char> spec_out;
+    bool mixed_precision = false;        // FP32 radiative transfer in the volumetric path
 
     // Per-render buffers.
     grrt_amd::DevBuf<double> sample_rgb;

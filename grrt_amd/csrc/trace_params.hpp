@@ -24,6 +24,20 @@ struct DevCounters {
     unsigned long long pad[3];
 };
 
+// Pointers into device memory plus the scalars of one volumetric disk.
+struct VolDev {
+    double M, a;                         // the disk's own mass and spin (src/api.cpp:122-126)
+    double r_horizon, r_min, r_outer, r_isco, taper_w, E_isco, L_isco;
+    double rho_scale, noise_scale, turbulence;
+    int octaves, n_r, n_z;
+    const double *H, *z_max, *rho_mid, *rho_prof, *T_prof;   // n_r, n_r*n_z
+    // Opacity tables: ln(max(kappa_abs, 1e-100)) [nu][rho][T] and kappa_es [rho][T].
+    const double *ln_kabs, *kes;
+    int n_nu, n_rho, n_T;
+    double lnu0, lnu1, lrho0, lrho1, lT0, lT1;
+    const int* perm;                     // 512 entries
+};
+
 struct TraceParams {
     // Kerr metric in Boyer–Lindquist coordinates.
     double M, a;
@@ -66,6 +80,14 @@ struct TraceParams {
 
     int refill_min;                     // idle lanes that trigger a refill
     int chunk;                          // rays claimed per queue atomic (>= 64)
+
+    // Volumetric disk (the volumetric tracer only).
+    VolDev vol;
+    int nbins;                          // 3 for RGB, else spectral bins
+    const double* nu_obs;               // observer-frame frequencies [Hz]
+    const double* lognu_obs;            // log10 of the same
+    void* rt_scratch;                   // spectral accumulators, 4 * nbins * n_rays of RT
+    double* sample_spec;                // n_rays * nbins spectral samples
 };
 
 }  // namespace grrt_amd

@@ -22,6 +22,8 @@
 
 #include "device_math.hpp"
 #include "kerr_rhs.hpp"
+#include "camera_device.hpp"
+#include "star_device.hpp"
 #include "trace_params.hpp"
 
 namespace grrt_amd {
@@ -67,64 +69,10 @@ __device__ __forceinline__ double hamiltonian_from_rhs(const Ray& R, double dt) 
     return 0.5 * (R.P0 * dt + R.y.v[3] * R.k1.v[0] + R.y.v[4] * R.k1.v[1] + R.P3 * R.k1.v[2]);
 }
 
-// Fractional pixel position of ray `id` (src/renderer.cpp:48-55 and the
-// extension sampling modes).
-__device__ __forceinline__ void sample_position(const TraceParams& P, long long id, double& px,
-                                                double& py) {
-    const long long pl = id / P.nS;
-    const int s = static_cast<int>(id - pl * P.nS);
-    const int i = P.x0 + static_cast<int>(pl % P.tw);
-    const int j = P.y0 + static_cast<int>(pl / P.tw);
-    if (P.mode == GRRT_SAMPLING_CENTER) {
-        px = i + 0.5;
-        py = j + 0.5;
-    } else if (P.mode == GRRT_SAMPLING_HASH) {
-        auto hash = [](int ii, int jj, int ss, int ch) -> double {
-            uint32_t h = static_cast<uint32_t>(ii * 73856093u ^ jj * 19349663u ^ ss * 83492791u ^
-                                               ch * 45678917u);
-            h ^= h >> 16;
-            h *= 0x45d9f3bu;
-            h ^= h >> 16;
-            return (h & 0xFFFFu) / 65536.0;
-        };
-        const int sy = s / P.grid, sx = s - sy * P.grid;
-        px = i + (sx + hash(i, j, s, 0)) * P.cell;
-        py = j + (sy + hash(i, j, s, 1)) * P.cell;
-    } else {
-        // Seeded: pixel-major global sample index over the full frame.
-        const long long gid = (static_cast<long long>(j) * P.W + i) * P.nS + s;
-        const uint64_t key = P.seed * 0xD1B54A32D192ED03ull;
-        const uint64_t w = 0x9E3779B97F4A7C15ull;
-        const uint64_t u = static_cast<uint64_t>(gid) * 2u;
-        const double jx = static_cast<double>(mix64(key + (u + 1u) * w) >> 11) * 0x1.0p-53;
-        const double jy = static_cast<double>(mix64(key + (u + 2u) * w) >> 11) * 0x1.0p-53;
-        px = i + jx;
-        py = j + jy;
-    }
-}
-
-// Camera ray (src/camera.cpp:69-96): p^mu = -e0 + d, lowered with g at the
-// observer; only the non-zero metric entries enter.
+// Ray start: camera covector (src/camera.cpp:69-96) and the first
+// right-hand side.
 __device__ __forceinline__ void init_ray(const TraceParams& P, const KerrK& K, long long id, Ray& R) {
-    double px, py;
-    sample_position(P, id, px, py);
-    const double alpha = (px - P.W / 2.0) * P.fov / P.W;
-    const double beta = (py - P.H / 2.0) * P.fov / P.W;
-    double sa, ca_, sb, cb;
-    sincos(alpha, &sa, &ca_);
-    sincos(beta, &sb, &cb);
-    const double ca = cb * sa, cc = cb * ca_;
-    double pc[4];
-    pc[0] = -P.e0t + (-ca * P.e1[0] - sb * P.e2[0] + cc * P.e3[0]);
-#pragma unroll
-    for (int m = 1; m < 4; ++m) pc[m] = -ca * P.e1[m] - sb * P.e2[m] + cc * P.e3[m];
-    R.P0 = P.g00 * pc[0] + P.g03 * pc[3];
-    R.P3 = P.g03 * pc[0] + P.g33 * pc[3];
-    R.y.v[0] = P.r_obs;
-    R.y.v[1] = P.th_obs;
-    R.y.v[2] = P.ph_obs;
-    R.y.v[3] = P.g11 * pc[1];
-    R.y.v[4] = P.g22 * pc[2];
+    camera_ray(P, id, R.P0, R.P3, R.y);
     R.dl = 0.01 * P.r_obs;
     R.col[0] = R.col[1] = R.col[2] = 0.0;
     R.r_hit = R.g_hit = 0.0;
@@ -181,43 +129,6 @@ __device__ __forceinline__ void disk_emission(const TraceParams& P, const double
                               : s_spec[3 * idx + ch] * (1.0 - t) + s_spec[3 * (idx + 1) + ch] * t;
         R.col[ch] += cv * wgt;
     }
-}
-
-// Nearest star within 0.003 rad in the 3x3 neighbouring cells, first in
-// cell-then-catalogue order (src/celestial_sphere.cpp:48-90).
-__device__ double star_sample(const TraceParams& P, double theta, double phi) {
-    const double twopi = 2.0 * kPi;
-    theta = fmod(theta, twopi);
-    if (theta < 0.0) theta += twopi;
-    if (theta > kPi) {
-        theta = twopi - theta;
-        phi += kPi;
-    }
-    phi = fmod(phi, twopi);
-    if (phi > kPi) phi -= twopi;
-    if (phi < -kPi) phi += twopi;
-    const int tb = static_cast<int>(theta * kStarGT / kPi);
-    const int pb = static_cast<int>((phi + kPi) * kStarGP / twopi);
-    const double st = sin(theta);
-    for (int dt = -1; dt <= 1; ++dt) {
-        int t = tb + dt;
-        t = t < 0 ? 0 : (t > kStarGT - 1 ? kStarGT - 1 : t);
-        for (int dp = -1; dp <= 1; ++dp) {
-            const int p = (((pb + dp) % kStarGP) + kStarGP) % kStarGP;
-            const int cell = t * kStarGP + p;
-            const int e = P.star_start[cell + 1];
-            for (int k = P.star_start[cell]; k < e; ++k) {
-                const double sth = P.star_tpb[3 * k], sph = P.star_tpb[3 * k + 1];
-                const double dth = theta - sth;
-                double dph = phi - sph;
-                if (dph > kPi) dph -= twopi;
-                if (dph < -kPi) dph += twopi;
-                const double d2 = dth * dth + dph * dph * st * st;
-                if (d2 < 0.003 * 0.003) return P.star_tpb[3 * k + 2];
-            }
-        }
-    }
-    return 0.0;
 }
 
 __device__ __forceinline__ void finish_ray(const TraceParams& P, long long id, Ray& R, int term) {

@@ -1,0 +1,423 @@
+// Volumetric-disk tracer for gfx950: one lane per ray.
+//
+// Per-ray semantics follow the reference:
+//   src/geodesic_tracer.cpp:23-181    outer loop, step clamp near the disk,
+//                                     entry detection (RGB mode)
+//   src/geodesic_tracer.cpp:183-337   RGB ray-march (450/550/650 nm)
+//   src/geodesic_tracer.cpp:495-603   spectral outer loop
+//   src/geodesic_tracer.cpp:605-746   N-bin spectral ray-march
+// The geodesic is integrated in FP64 (the same Dormand–Prince step as the
+// thin-disk kernel; classic RK4 inside the ray-march).  Radiative transfer
+// runs in the template type RT: double for the all-FP64 path, float for the
+// mixed-precision path, where each bin's intensity is accumulated with
+// Neumaier compensated summation.
+#include <hip/hip_runtime.h>
+
+#include "device_math.hpp"
+#include "kerr_rhs.hpp"
+#include "camera_device.hpp"
+#include "star_device.hpp"
+#include "trace_params.hpp"
+#include "volumetric_device.hpp"
+
+namespace grrt_amd {
+
+namespace {
+
+constexpr double A21 = 1.0 / 5.0;
+constexpr double A31 = 3.0 / 40.0, A32 = 9.0 / 40.0;
+constexpr double A41 = 44.0 / 45.0, A42 = -56.0 / 15.0, A43 = 32.0 / 9.0;
+constexpr double A51 = 19372.0 / 6561.0, A52 = -25360.0 / 2187.0, A53 = 64448.0 / 6561.0,
+                 A54 = -212.0 / 729.0;
+constexpr double A61 = 9017.0 / 3168.0, A62 = -355.0 / 33.0, A63 = 46732.0 / 5247.0,
+                 A64 = 49.0 / 176.0, A65 = -5103.0 / 18656.0;
+constexpr double B1 = 35.0 / 384.0, B3 = 500.0 / 1113.0, B4 = 125.0 / 192.0,
+                 B5 = -2187.0 / 6784.0, B6 = 11.0 / 84.0;
+constexpr double E1 = 71.0 / 57600.0, E3 = -71.0 / 16695.0, E4 = 71.0 / 1920.0,
+                 E5 = -17253.0 / 339200.0, E6 = 22.0 / 525.0, E7 = -1.0 / 40.0;
+
+struct GState {
+    Vec5 y;         // r, theta, phi, p_r, p_theta
+    double P0, P3;  // conserved p_t, p_phi
+};
+
+__device__ __forceinline__ double step_factor_v(double x) {
+    const float lx = __builtin_amdgcn_logf(static_cast<float>(x));
+    return static_cast<double>(0.9f * __builtin_amdgcn_exp2f(0.2f * lx));
+}
+
+// One adaptive Dormand–Prince step (src/rk4.cpp:279-328); returns next dl,
+// counts attempts.
+__device__ double adaptive_dp45(const KerrK& K, GState& s, double dl, double tol, int& attempts) {
+    double dtmp;
+    const Vec5 k1 = kerr_rhs(K, s.P0, s.P3, s.y, dtmp);
+    for (int retry = 0; retry <= 20; ++retry) {
+        const double h = dl;
+        Vec5 st, k2, k3, k4, k5, k6, k7, y5, e;
+        for (int i = 0; i < 5; ++i) st.v[i] = __builtin_fma(k1.v[i], h * A21, s.y.v[i]);
+        k2 = kerr_rhs(K, s.P0, s.P3, st, dtmp);
+        for (int i = 0; i < 5; ++i)
+            st.v[i] = __builtin_fma(k2.v[i], h * A32, __builtin_fma(k1.v[i], h * A31, s.y.v[i]));
+        k3 = kerr_rhs(K, s.P0, s.P3, st, dtmp);
+        for (int i = 0; i < 5; ++i)
+            st.v[i] = __builtin_fma(k3.v[i], h * A43, __builtin_fma(k2.v[i], h * A42, __builtin_fma(k1.v[i], h * A41, s.y.v[i])));
+        k4 = kerr_rhs(K, s.P0, s.P3, st, dtmp);
+        for (int i = 0; i < 5; ++i)
+            st.v[i] = __builtin_fma(k4.v[i], h * A54, __builtin_fma(k3.v[i], h * A53,
+                          __builtin_fma(k2.v[i], h * A52, __builtin_fma(k1.v[i], h * A51, s.y.v[i]))));
+        k5 = kerr_rhs(K, s.P0, s.P3, st, dtmp);
+        for (int i = 0; i < 5; ++i)
+            st.v[i] = __builtin_fma(k5.v[i], h * A65, __builtin_fma(k4.v[i], h * A64,
+                          __builtin_fma(k3.v[i], h * A63, __builtin_fma(k2.v[i], h * A62,
+                              __builtin_fma(k1.v[i], h * A61, s.y.v[i])))));
+        k6 = kerr_rhs(K, s.P0, s.P3, st, dtmp);
+        for (int i = 0; i < 5; ++i) {
+            y5.v[i] = __builtin_fma(k6.v[i], h * B6, __builtin_fma(k5.v[i], h * B5,
+                          __builtin_fma(k4.v[i], h * B4, __builtin_fma(k3.v[i], h * B3,
+                              __builtin_fma(k1.v[i], h * B1, s.y.v[i])))));
+            e.v[i] = __builtin_fma(k6.v[i], h * E6, __builtin_fma(k5.v[i], h * E5,
+                         __builtin_fma(k4.v[i], h * E4, __builtin_fma(k3.v[i], h * E3, k1.v[i] * (h * E1)))));
+        }
+        k7 = kerr_rhs(K, s.P0, s.P3, y5, dtmp);
+        double err = 0.0;
+        for (int i = 0; i < 5; ++i)
+            err = __builtin_fmax(err, __builtin_fabs(__builtin_fma(k7.v[i], h * E7, e.v[i])) *
+                                          rcp1(__builtin_fabs(y5.v[i]) + 1e-10));
+        ++attempts;
+        if (retry == 20) {
+            s.y = y5;
+            return h;
+        }
+        if (err <= tol) {
+            double f = (err > 1e-30) ? step_factor_v(tol / err) : 5.0;
+            f = sclamp(f, 1.0, 5.0);
+            s.y = y5;
+            return smin(h * f, 5.0 * smax(__builtin_fabs(y5.v[0]), 1.0));
+        }
+        dl = h * sclamp(step_factor_v(tol / err), 0.2, 1.0);
+        if (dl < 1e-6) {
+            s.y = y5;
+            return smin(1e-6, 5.0 * smax(__builtin_fabs(y5.v[0]), 1.0));
+        }
+    }
+    return dl;   // not reached
+}
+
+// Classic RK4 step (include/grrt/geodesic/rk4.h:34-52).
+__device__ GState rk4_step(const KerrK& K, const GState& s, double h) {
+    double dt;
+    GState t = s, o = s;
+    const Vec5 k1 = kerr_rhs(K, s.P0, s.P3, s.y, dt);
+    for (int i = 0; i < 5; ++i) t.y.v[i] = s.y.v[i] + k1.v[i] * (h * 0.5);
+    const Vec5 k2 = kerr_rhs(K, s.P0, s.P3, t.y, dt);
+    for (int i = 0; i < 5; ++i) t.y.v[i] = s.y.v[i] + k2.v[i] * (h * 0.5);
+    const Vec5 k3 = kerr_rhs(K, s.P0, s.P3, t.y, dt);
+    for (int i = 0; i < 5; ++i) t.y.v[i] = s.y.v[i] + k3.v[i] * h;
+    const Vec5 k4 = kerr_rhs(K, s.P0, s.P3, t.y, dt);
+    for (int i = 0; i < 5; ++i)
+        o.y.v[i] = s.y.v[i] + (k1.v[i] + k2.v[i] * 2.0 + k3.v[i] * 2.0 + k4.v[i]) * (h / 6.0);
+    return o;
+}
+
+// Per-ray radiative-transfer accumulators: J in units of 2h/c^2 (with
+// Neumaier compensation Jc when RT is float), transmission T and
+// accumulated optical depth tau.  RGB mode keeps its three channels in
+// registers; spectral mode keeps N bins in device scratch laid out
+// [quantity][bin][ray] so one bin of a wavefront is contiguous.
+template <class RT, bool kSpectral>
+struct Bins;
+
+template <class RT>
+struct Bins<RT, false> {
+    RT Jv[3], Jcv[3], Tv[3], tauv[3];
+    __device__ Bins(const TraceParams&, long long) {
+        for (int c = 0; c < 3; ++c) { Jv[c] = RT(0); Jcv[c] = RT(0); Tv[c] = RT(1); tauv[c] = RT(0); }
+    }
+    __device__ RT& J(int c) { return Jv[c]; }
+    __device__ RT& Jc(int c) { return Jcv[c]; }
+    __device__ RT& T(int c) { return Tv[c]; }
+    __device__ RT& tau(int c) { return tauv[c]; }
+};
+
+template <class RT>
+struct Bins<RT, true> {
+    RT *Jp, *Jcp, *Tp, *taup;
+    long long stride;
+    __device__ Bins(const TraceParams& P, long long id) {
+        RT* base = static_cast<RT*>(P.rt_scratch);
+        const long long plane = static_cast<long long>(P.nbins) * P.n_rays;
+        stride = P.n_rays;
+        Jp = base + id;
+        Jcp = base + plane + id;
+        Tp = base + 2 * plane + id;
+        taup = base + 3 * plane + id;
+        for (int c = 0; c < P.nbins; ++c) {
+            Jp[c * stride] = RT(0);
+            Jcp[c * stride] = RT(0);
+            Tp[c * stride] = RT(1);
+            taup[c * stride] = RT(0);
+        }
+    }
+    __device__ RT& J(int c) { return Jp[c * stride]; }
+    __device__ RT& Jc(int c) { return Jcp[c * stride]; }
+    __device__ RT& T(int c) { return Tp[c * stride]; }
+    __device__ RT& tau(int c) { return taup[c * stride]; }
+};
+
+// Compensated addition (Neumaier 1974): s + c carries the exact running sum.
+__device__ __forceinline__ void neumaier(float& s, float& c, float y) {
+    const float t = s + y;
+    c += (fabsf(s) >= fabsf(y)) ? (s - t) + y : (y - t) + s;
+    s = t;
+}
+__device__ __forceinline__ void neumaier(double& s, double&, double y) { s += y; }
+
+struct MarchOut {
+    int emitted;      // at least one emitting sample
+    int steps;        // RK4 steps taken
+};
+
+// Backward ray-march through the volume from `st`; advances `st` to the exit
+// state.  RGB mode: channels at 450/550/650 nm, step control on 550 nm and
+// an opacity exit; spectral mode: step control on bin N/2 and an exit once
+// every bin is past tau = 10.
+template <class RT, bool kSpectral>
+__device__ MarchOut raymarch(const TraceParams& P, const int* perm, const KerrK& K, GState& st,
+                             Bins<RT, kSpectral>& b) {
+    const VolDev& V = P.vol;
+    const int nb = kSpectral ? P.nbins : 3;
+    const int med = kSpectral ? P.nbins / 2 : 1;
+    const double ut_obs = 1.0 / sqrt(1.0 - 2.0 / P.r_obs);
+    MarchOut mo{0, 0};
+    double r = st.y.v[0];
+    const double z0 = r * cos(st.y.v[1]);
+    const double H0 = vol_H(V, r);
+    bool been_inside = vol_inside(V, r, z0);
+    double ds = been_inside ? H0 / 16.0 : smin(__builtin_fabs(z0) / 8.0, H0 * 2.0);
+    // Exit tests use the bins' running extreme values.
+    RT ext = kSpectral ? RT(1e30) : RT(0);
+    for (int c = 0; c < nb; ++c)
+        ext = kSpectral ? (b.tau(c) < ext ? b.tau(c) : ext) : (b.T(c) > ext ? b.T(c) : ext);
+    for (int n = 0; n < 4096; ++n) {
+        const GState ns = rk4_step(K, st, ds);
+        ++mo.steps;
+        r = ns.y.v[0];
+        const double th = ns.y.v[1], ph = ns.y.v[2];
+        const double z = r * cos(th);
+        if (r < V.r_horizon || r > V.r_outer) { st = ns; break; }
+        if (kSpectral ? (ext > RT(10)) : (ext < RT(1e-6))) { st = ns; break; }
+        const double H = vol_H(V, r);
+        if (!vol_inside(V, r, z)) {
+            const double zm = vol_zmax(V, r);
+            if (been_inside && __builtin_fabs(z) > zm + 3.0 * H) { st = ns; break; }
+            if (!been_inside) ds = smax(smin(__builtin_fabs(z) / 8.0, H * 2.0), H / 64.0);
+            else ds = sclamp(H / 4.0, H / 64.0, H);
+            st = ns;
+            continue;
+        }
+        been_inside = true;
+        const double rho = vol_density(V, perm, r, z, ph);
+        const double Tmp = vol_temperature(V, r, __builtin_fabs(z));
+        if (rho <= 0.0 || Tmp <= 0.0) { st = ns; continue; }
+        double ut, ur, up;
+        vol_velocity(V, r, th, ut, ur, up);
+        const double pue = ns.P0 * ut + ns.y.v[3] * ur + ns.P3 * up;
+        const double g = pue / (ns.P0 * ut_obs);
+        const double ag = __builtin_fabs(g);
+        const double lag = log10(ag);
+        const RT dsp = static_cast<RT>(__builtin_fabs(pue) * __builtin_fabs(ds));
+        const OpCell<RT> cell = vol_op_cell<RT>(V, rho, Tmp);
+        const RT kes = vol_kes<RT>(V, cell);
+        const RT rhoT = static_cast<RT>(rho), TT = static_cast<RT>(Tmp);
+        RT kmed = RT(0);
+        ext = kSpectral ? RT(1e30) : RT(0);
+        for (int c = 0; c < nb; ++c) {
+            const double nu_o = kSpectral ? P.nu_obs[c] : (c == 0 ? 2.997924e10 / 450e-7 : (c == 1 ? 2.997924e10 / 550e-7 : 2.997924e10 / 650e-7));
+            const double lnu_o = kSpectral ? P.lognu_obs[c] : log10(nu_o);
+            const RT nue = static_cast<RT>(ag * nu_o);
+            const RT kabs = vol_kabs<RT>(V, cell, lag + lnu_o);
+            if (c == med) kmed = kabs;
+            const RT ktot = kabs + kes;
+            const RT eps = (ktot > RT(0)) ? kabs / ktot : RT(1);
+            const RT dtau = ktot * rhoT * dsp;
+            const RT src = eps * vol_source(nue, TT);   // S = eps B_nu(nu_e) / nu_e^3 in units of 2h/c^2
+            const RT ex = exp(-dtau);
+            RT& Tc = b.T(c);
+            neumaier(b.J(c), b.Jc(c), Tc * src * (RT(1) - ex));
+            Tc *= ex;
+            if (kSpectral) {
+                RT& tc = b.tau(c);
+                tc += dtau;
+                ext = tc < ext ? tc : ext;
+            } else {
+                ext = Tc > ext ? Tc : ext;
+            }
+        }
+        const double alpha_tot = static_cast<double>((kmed + kes) * rhoT);
+        const double ds_tau = alpha_tot > 0.0 ? 0.05 / alpha_tot : ds * 2.0;
+        const double ds_geo = 0.1 * smax(r - V.r_horizon, 0.5);
+        ds = sclamp(smin(ds_tau, ds_geo), H / 64.0, H);
+        st = ns;
+        mo.emitted = 1;
+    }
+    return mo;
+}
+
+}  // namespace
+
+// One lane per ray: outer Dormand–Prince loop with the near-disk step clamp
+// and volume entry detection, ray-marching on every entry.
+template <class RT, bool kSpectral>
+__global__ __launch_bounds__(256) void trace_vol(const TraceParams* Pg) {
+    __shared__ int s_perm[512];
+    for (int i = threadIdx.x; i < 512; i += blockDim.x) s_perm[i] = Pg->vol.perm[i];
+    __syncthreads();
+    const TraceParams& P = *Pg;
+    // Not a persistent kernel: every ray is claimed by its launch.
+    if (blockIdx.x == 0 && threadIdx.x == 0) P.counters->queue_head = static_cast<unsigned long long>(P.n_rays);
+    const long long id = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (id >= P.n_rays) return;
+    const VolDev& V = P.vol;
+    const KerrK K{P.M, P.a, P.a * P.a, 2.0 * P.M};
+    GState s;
+    camera_ray(P, id, s.P0, s.P3, s.y);
+    Bins<RT, kSpectral> b(P, id);
+    const int nb = kSpectral ? P.nbins : 3;
+    const double half_pi = kPi / 2.0;
+    double dl = 0.01 * P.r_obs;
+    int term = 2, steps = 0, attempts = 0, entries = 0, emitted = 0, march_steps = 0;
+    for (int step = 0; step < P.max_steps; ++step) {
+        const double r = s.y.v[0];
+        if (r < P.r_term) { term = 0; break; }
+        if (r > P.r_escape) { term = 1; break; }
+        const GState prev = s;
+        {
+            const double th = s.y.v[1];
+            const double z = r * cos(th);
+            if (r >= V.r_horizon * 0.9 && r <= V.r_outer * 1.5) {
+                const double H = vol_H(V, r), zm = vol_zmax(V, r);
+                if (__builtin_fabs(z) < zm + 6.0 * H) {
+                    double dt;
+                    const Vec5 d = kerr_rhs(K, s.P0, s.P3, s.y, dt);
+                    const double dz = __builtin_fabs(cos(th) * d.v[0] - r * sin(th) * d.v[1]);
+                    if (dz > 1e-20) dl = smin(dl, smax(0.25 * H / dz, H / 64.0));
+                }
+            }
+        }
+        dl = adaptive_dp45(K, s, dl, P.tol, attempts);
+        ++steps;
+        bool opaque = false;
+        if (!kSpectral) {
+            opaque = true;
+            for (int c = 0; c < 3; ++c) opaque = opaque && (b.T(c) < RT(1e-6));
+        }
+        if (!opaque) {
+            const double tp = prev.y.v[1], tn = s.y.v[1];
+            const double dp = tp - half_pi, dn = tn - half_pi;
+            const double rn = s.y.v[0], rp = prev.y.v[0];
+            const double zn = rn * cos(tn), zp = rp * cos(tp);
+            const bool crossed = (dp * dn < 0.0) && __builtin_fabs(dp - dn) > 1e-12;
+            const bool inside_now = vol_inside(V, rn, zn);
+            const bool near = (__builtin_fabs(zn) < vol_zmax(V, rn) + 2.0 * vol_H(V, rn) ||
+                               __builtin_fabs(zp) < vol_zmax(V, rp) + 2.0 * vol_H(V, rp)) &&
+                              rn >= V.r_horizon && rn <= V.r_outer;
+            if (crossed || inside_now || near) {
+                const double rlo = smin(rp, rn), rhi = smax(rp, rn);
+                if (rhi >= V.r_horizon && rlo <= V.r_outer && rp >= V.r_horizon * 0.9 && rp <= V.r_outer * 1.5) {
+                    GState entry = prev;
+                    const MarchOut mo = raymarch<RT, kSpectral>(P, s_perm, K, entry, b);
+                    emitted |= mo.emitted;
+                    march_steps += mo.steps;
+                    ++entries;
+                    s = entry;
+                    continue;
+                }
+            }
+        }
+    }
+    // I_obs = J nu_obs^3 (2h/c^2 restores CGS units).
+    constexpr double two_h_c2 = 2.0 * 6.626070e-27 / (2.997924e10 * 2.997924e10);
+    double col[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < nb; ++c) {
+        const double nu_o = kSpectral ? P.nu_obs[c] : (c == 0 ? 2.997924e10 / 450e-7 : (c == 1 ? 2.997924e10 / 550e-7 : 2.997924e10 / 650e-7));
+        const double Jt = static_cast<double>(b.J(c)) + static_cast<double>(b.Jc(c));
+        const double I = Jt * two_h_c2 * nu_o * nu_o * nu_o;
+        if (kSpectral) P.sample_spec[id * nb + c] = I;
+        else col[c] = I;
+    }
+    if (!kSpectral) {
+        if (term == 1 && P.stars) {
+            // stars: the thin tracer's catalogue search (src/renderer.cpp:56-61)
+            const double bgt = star_sample(P, s.y.v[1], s.y.v[2]);
+            for (int c = 0; c < 3; ++c) col[c] += bgt;
+        }
+        for (int c = 0; c < 3; ++c) P.sample_rgb[3 * id + c] = col[c];
+    }
+    const int cls = emitted ? GRRT_CLASS_DISK
+                            : (term == 0 ? GRRT_CLASS_CAPTURED : (term == 1 ? GRRT_CLASS_ESCAPED : GRRT_CLASS_MAXSTEPS));
+    P.sample_cls[id] = static_cast<uint8_t>(cls);
+    if (P.records) {
+        GRRTRayRecord& o = P.records[id];
+        o.color[0] = col[0];
+        o.color[1] = col[1];
+        o.color[2] = col[2];
+        o.x_final[0] = 0.0;
+        o.x_final[1] = s.y.v[0];
+        o.x_final[2] = s.y.v[1];
+        o.x_final[3] = s.y.v[2];
+        o.p_final[0] = s.P0;
+        o.p_final[1] = s.y.v[3];
+        o.p_final[2] = s.y.v[4];
+        o.p_final[3] = s.P3;
+        o.r_hit = o.g_hit = 0.0;
+        o.h_max = o.h_final = 0.0;
+        o.term = term;
+        o.cls = cls;
+        o.steps = steps;
+        o.attempts = attempts;
+        o.n_cross = entries;
+        o.pad = march_steps;
+    }
+    atomicAdd(&P.counters->steps, static_cast<unsigned long long>(steps));
+    atomicAdd(&P.counters->attempts, static_cast<unsigned long long>(attempts));
+    atomicAdd(&P.counters->rays_done, 1ull);
+}
+
+// Average the N-bin spectra of each pixel (src/renderer.cpp:235-266); the
+// output is double (ABI) or float (cube) per `out_f32`.
+__global__ void finalize_spectral(const double* __restrict__ spec, void* out, long long npix, int nS,
+                                  int nb, double inv, int out_f32) {
+    const long long q = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (q >= npix * nb) return;
+    const long long p = q / nb;
+    const int k = static_cast<int>(q - p * nb);
+    double acc = 0.0;
+    for (int s = 0; s < nS; ++s) acc += spec[(p * nS + s) * nb + k];
+    if (out_f32) static_cast<float*>(out)[q] = static_cast<float>(acc * inv);
+    else static_cast<double*>(out)[q] = acc * inv;
+}
+
+hipError_t launch_trace_vol(const TraceParams* P, long long n_rays, bool spectral, bool fp32_rt,
+                            hipStream_t st) {
+    const int T = 256;
+    const dim3 grid(static_cast<unsigned>((n_rays + T - 1) / T));
+    if (spectral) {
+        if (fp32_rt) hipLaunchKernelGGL((trace_vol<float, true>), grid, dim3(T), 0, st, P);
+        else hipLaunchKernelGGL((trace_vol<double, true>), grid, dim3(T), 0, st, P);
+    } else {
+        if (fp32_rt) hipLaunchKernelGGL((trace_vol<float, false>), grid, dim3(T), 0, st, P);
+        else hipLaunchKernelGGL((trace_vol<double, false>), grid, dim3(T), 0, st, P);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize_spectral(const double* spec, void* out, long long npix, int nS, int nb, bool f32,
+                                    hipStream_t st) {
+    const int T = 256;
+    const long long n = npix * nb;
+    hipLaunchKernelGGL(finalize_spectral, dim3(static_cast<unsigned>((n + T - 1) / T)), dim3(T), 0, st, spec,
+                       out, npix, nS, nb, 1.0 / nS, f32 ? 1 : 0);
+    return hipGetLastError();
+}
+
+}  // namespace grrt_amd

@@ -74,6 +74,34 @@ GRRT_EXPORT int grrt_ext_render_device(GRRTContext* ctx, float* d_rgba);
 GRRT_EXPORT int grrt_ext_set_tuning(GRRTContext* ctx, int block, int blocks_per_cu, int refill_min,
                                     int chunk);
 
+/* Volumetric path precision: 0 = all FP64 (default), 1 = FP32 radiative
+ * transfer (opacities, source, extinction) with compensated summation of
+ * each bin's intensity; the geodesic stays FP64. */
+GRRT_EXPORT int grrt_ext_set_precision(GRRTContext* ctx, int mode);
+/* grrt_render_spectral into a float32 cube (width*height*nbins). */
+GRRT_EXPORT int grrt_ext_render_spectral_f32(GRRTContext* ctx, float* out, int width, int height);
+
+/* Host-only access to the volumetric disk set-up (no GPU needed): builds
+ * the opacity, radial and vertical tables for the volumetric disk that
+ * grrt_create would build from `params`. */
+typedef struct GRRTVolTables GRRTVolTables;
+GRRT_EXPORT GRRTVolTables* grrt_ext_vol_build(const GRRTParams* params);
+/* Scalars in this order: r_isco, r_horizon, r_min, r_outer, taper_width,
+ * E_isco, L_isco, rho_scale, noise_scale, turbulence, octaves, n_r, n_z,
+ * n_nu, log10 nu_min, log10 nu_max, M, a, t_peak.  Returns the count. */
+GRRT_EXPORT int grrt_ext_vol_scalars(const GRRTVolTables* t, double* out, int cap);
+/* Copies table `name` ("H", "z_max", "rho_mid", "T_eff", "rho_profile",
+ * "T_profile", "kappa_abs", "kappa_es", "kappa_ross", "mu", "perm") into
+ * `out` if it fits; returns the table length, or -1 for an unknown name. */
+GRRT_EXPORT int64_t grrt_ext_vol_table(const GRRTVolTables* t, const char* name, double* out, int64_t cap);
+GRRT_EXPORT void grrt_ext_vol_free(GRRTVolTables* t);
+/* Saha ionisation of the H/He/C/O/Fe mixture: n_e, mu, n_ion_eff, n_HI,
+ * n_Hminus (CGS). */
+GRRT_EXPORT int grrt_ext_saha(double rho_cgs, double T, double out[5]);
+/* Point opacities at (nu, rho, T): kappa_abs, kappa_es [cm^2/g],
+ * alpha_ff, alpha_hminus, alpha_bf [1/cm], B_nu. */
+GRRT_EXPORT int grrt_ext_opacity(double nu, double rho_cgs, double T, double out[6]);
+
 #ifdef __cplusplus
 }
 #endif

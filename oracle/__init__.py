@@ -7,6 +7,7 @@ product library ``libgrrt.so`` never links or calls it.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -38,6 +39,46 @@ class OracleRay(C.Structure):
         ("term", C.c_int32), ("cls", C.c_int32),
         ("steps", C.c_int32), ("attempts", C.c_int32), ("n_cross", C.c_int32), ("pad", C.c_int32),
     ]
+
+
+class OracleVol(C.Structure):
+    _fields_ = [
+        ("M", C.c_double), ("a", C.c_double), ("r_horizon", C.c_double), ("r_min", C.c_double),
+        ("r_outer", C.c_double), ("r_isco", C.c_double), ("taper_w", C.c_double),
+        ("E_isco", C.c_double), ("L_isco", C.c_double), ("rho_scale", C.c_double),
+        ("noise_scale", C.c_double), ("turbulence", C.c_double),
+        ("octaves", C.c_int), ("n_r", C.c_int), ("n_z", C.c_int),
+        ("H", C.c_void_p), ("z_max", C.c_void_p), ("rho_mid", C.c_void_p),
+        ("rho_prof", C.c_void_p), ("T_prof", C.c_void_p), ("kabs", C.c_void_p), ("kes", C.c_void_p),
+        ("n_nu", C.c_int), ("n_rho", C.c_int), ("n_T", C.c_int),
+        ("lnu0", C.c_double), ("lnu1", C.c_double), ("lrho0", C.c_double), ("lrho1", C.c_double),
+        ("lT0", C.c_double), ("lT1", C.c_double),
+        ("perm", C.c_void_p),
+    ]
+
+
+def vol_struct(T: dict) -> OracleVol:
+    """OracleVol over the tables of a volumetric disk (a dict of numpy arrays
+    and scalars as produced by grrt_amd.volumetric_tables); the arrays are
+    kept alive on the returned object."""
+    keep = {k: np.ascontiguousarray(T[k], np.float64) for k in
+            ("H", "z_max", "rho_mid", "rho_profile", "T_profile", "kappa_abs", "kappa_es")}
+    keep["perm"] = np.ascontiguousarray(T["perm"], np.int32)
+    v = OracleVol()
+    for f in ("r_isco", "r_min", "r_outer", "E_isco", "L_isco", "rho_scale", "noise_scale",
+              "turbulence", "M", "a"):
+        setattr(v, f, T[f])
+    v.r_horizon = T["r_horizon"]
+    v.taper_w = T["taper_width"]
+    v.octaves, v.n_r, v.n_z = int(T["octaves"]), int(T["n_r"]), int(T["n_z"])
+    v.H, v.z_max, v.rho_mid = keep["H"].ctypes.data, keep["z_max"].ctypes.data, keep["rho_mid"].ctypes.data
+    v.rho_prof, v.T_prof = keep["rho_profile"].ctypes.data, keep["T_profile"].ctypes.data
+    v.kabs, v.kes, v.perm = keep["kappa_abs"].ctypes.data, keep["kappa_es"].ctypes.data, keep["perm"].ctypes.data
+    v.n_nu, v.n_rho, v.n_T = int(T["n_nu"]), 100, 100
+    v.lnu0, v.lnu1 = T["log_nu_min"], T["log_nu_max"]
+    v.lrho0, v.lrho1, v.lT0, v.lT1 = -18.0, -6.0, math.log10(3000.0), 8.0
+    v._keep = keep
+    return v
 
 
 RAY_DTYPE = np.dtype([
@@ -95,6 +136,18 @@ def lib(variant: str = "strict") -> C.CDLL:
     L.oracle_render.argtypes = [C.POINTER(OracleScene), C.c_int, C.c_int, C.c_uint64,
                                 C.POINTER(C.c_float), C.POINTER(C.c_uint8), C.c_int]
     L.oracle_render.restype = C.c_int64
+    L.oracle_vol_trace_samples.argtypes = [C.POINTER(OracleScene), C.POINTER(OracleVol), C.c_int, C.c_int,
+                                           C.c_uint64, C.POINTER(C.c_int64), C.c_int64, C.c_int,
+                                           _PD, C.c_void_p, _PD, C.c_int]
+    for fn in ("oracle_vol_density", "oracle_vol_temperature"):
+        getattr(L, fn).argtypes = [C.POINTER(OracleVol), _D, _D] + ([_D] if fn.endswith("density") else [])
+        getattr(L, fn).restype = _D
+    L.oracle_vol_kappa_abs.argtypes = [C.POINTER(OracleVol), _D, _D, _D]
+    L.oracle_vol_kappa_abs.restype = _D
+    L.oracle_vol_kappa_es.argtypes = [C.POINTER(OracleVol), _D, _D]
+    L.oracle_vol_kappa_es.restype = _D
+    L.oracle_simplex.argtypes = [C.POINTER(C.c_int), _D, _D, _D]
+    L.oracle_simplex.restype = _D
     _LIBS[name] = L
     return L
 
@@ -248,3 +301,38 @@ def render(sc: OracleScene, mode: int = JITTER_CENTER, spp: int = 1, seed: int =
         C.byref(sc), mode, spp, seed, rgba.ctypes.data_as(C.POINTER(C.c_float)),
         cls.ctypes.data_as(C.POINTER(C.c_uint8)) if cls is not None else None, threads)
     return rgba, cls, steps
+
+
+def vol_trace_samples(sc: OracleScene, vol: OracleVol, mode: int, spp: int, seed: int, ids,
+                      freqs=None, threads=0, variant="strict"):
+    """Volumetric rays: returns (records, spectra or None)."""
+    ids = np.ascontiguousarray(ids, np.int64)
+    rays = np.zeros(len(ids), RAY_DTYPE)
+    nb = 0 if freqs is None else len(freqs)
+    nu = np.ascontiguousarray(freqs if freqs is not None else [0.0], np.float64)
+    spec = np.zeros((len(ids), max(nb, 1)))
+    lib(variant).oracle_vol_trace_samples(C.byref(sc), C.byref(vol), mode, spp, seed,
+                                          ids.ctypes.data_as(C.POINTER(C.c_int64)), len(ids), nb,
+                                          _dp(nu), rays.ctypes.data, _dp(spec), threads)
+    return rays, (spec if nb else None)
+
+
+def vol_density(vol, r, z, phi):
+    return lib().oracle_vol_density(C.byref(vol), r, z, phi)
+
+
+def vol_temperature(vol, r, z):
+    return lib().oracle_vol_temperature(C.byref(vol), r, z)
+
+
+def vol_kappa_abs(vol, nu, rho, T):
+    return lib().oracle_vol_kappa_abs(C.byref(vol), nu, rho, T)
+
+
+def vol_kappa_es(vol, rho, T):
+    return lib().oracle_vol_kappa_es(C.byref(vol), rho, T)
+
+
+def simplex(perm, x, y, z):
+    p = np.ascontiguousarray(perm, np.int32)
+    return lib().oracle_simplex(p.ctypes.data_as(C.POINTER(C.c_int)), x, y, z)

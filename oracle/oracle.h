@@ -46,6 +46,19 @@ typedef struct {
     int32_t steps, attempts, n_cross, pad;
 } OracleRay;
 
+/* Volumetric disk tables (built by the library's host set-up, which the
+ * tests pin separately) and scalars, consumed by the volumetric oracle. */
+typedef struct {
+    double M, a, r_horizon, r_min, r_outer, r_isco, taper_w, E_isco, L_isco;
+    double rho_scale, noise_scale, turbulence;
+    int octaves, n_r, n_z;
+    const double *H, *z_max, *rho_mid, *rho_prof, *T_prof;
+    const double *kabs, *kes;           /* kappa_abs [nu][rho][T], kappa_es [rho][T] */
+    int n_nu, n_rho, n_T;
+    double lnu0, lnu1, lrho0, lrho1, lT0, lT1;
+    const int* perm;                    /* 512 */
+} OracleVol;
+
 /* Sample placement modes for render/trace_rays. */
 enum { ORACLE_JITTER_HASH = 0, ORACLE_JITTER_CENTER = 1, ORACLE_JITTER_SEEDED = 2 };
 
@@ -79,6 +92,18 @@ void   oracle_trace_samples(const OracleScene* sc, int mode, int spp, uint64_t s
  * the number of accepted steps summed over all rays. */
 int64_t oracle_render(const OracleScene* sc, int mode, int spp, uint64_t seed,
                       float* rgba, uint8_t* cls, int threads);
+
+/* Volumetric path (src/geodesic_tracer.cpp:23-337 with a volumetric disk,
+ * :495-746 spectral).  nbins == 0: RGB ray-march, colours in rays[k].color;
+ * nbins > 0: spectral, spectra[k*nbins + b] = I_nu [erg/s/cm^2/Hz/sr]. */
+void   oracle_vol_trace_samples(const OracleScene* sc, const OracleVol* V, int mode, int spp,
+                                uint64_t seed, const int64_t* ids, int64_t n, int nbins,
+                                const double* nu, OracleRay* rays, double* spectra, int threads);
+double oracle_vol_density(const OracleVol* V, double r, double z, double phi);
+double oracle_vol_temperature(const OracleVol* V, double r, double z);
+double oracle_vol_kappa_abs(const OracleVol* V, double nu, double rho, double T);
+double oracle_vol_kappa_es(const OracleVol* V, double rho, double T);
+double oracle_simplex(const int* perm, double x, double y, double z);
 
 #ifdef __cplusplus
 }

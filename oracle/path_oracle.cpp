@@ -1,4 +1,5 @@
-// CPU oracle: restatement of the reference thin-disk geodesic path.
+// CPU oracle: restatement of the reference per-pixel geodesic path (thin
+// disk, volumetric disk and spectral ray-march).
 //
 // TEST INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and
 // bench.py's cpu_baseline leg as the checker; never by the product library.
@@ -174,8 +175,6 @@ double isco(double M, double a) {  // src/kerr.cpp:52-63 (Bardeen, Press & Teuko
 }
 
 // ------------------------------------------------------------ integrators ---
-inline void axpy_state(State& out, const State& s) { out = s; }
-
 // include/grrt/geodesic/rk4.h:73-177 — Dormand–Prince 5(4) with local
 // extrapolation; k7 evaluated at y5; relative max-norm error estimate.
 void dp45_attempt(double M, double a, const State& y, double dl, State& y5, double& err) {
@@ -678,7 +677,7 @@ int samples_per_pixel(int mode, int spp, int& grid) {
 }
 
 // Sample position for sample id (pixel*S + s); src/renderer.cpp:48-55.
-void sample_pos(const Scene& S, int mode, int grid, int nS, uint64_t seed, int64_t id,
+void sample_position_of(const Scene& S, int mode, int grid, int nS, uint64_t seed, int64_t id,
                 double& px, double& py) {
     const int64_t pixel = id / nS;
     const int s = static_cast<int>(id % nS);
@@ -697,6 +696,359 @@ void sample_pos(const Scene& S, int mode, int grid, int nS, uint64_t seed, int64
     seeded_jitter(seed, id, jx, jy);
     px = i + jx;
     py = j + jy;
+}
+
+
+// ------------------------------------------------------- volumetric disk ---
+constexpr double kC = 2.997924e10, kH = 6.626070e-27, kKB = 1.380649e-16;   // constants.h:11-35
+
+struct Vol {
+    const OracleVol& v;
+    // src/volumetric_disk.cpp:164-171
+    double interp_radial(const double* lut, double r) const {
+        if (r <= v.r_min) return lut[0];
+        if (r >= v.r_outer) return lut[v.n_r - 1];
+        const double frac = (r - v.r_min) / (v.r_outer - v.r_min) * (v.n_r - 1);
+        const int idx = std::clamp(static_cast<int>(frac), 0, v.n_r - 2);
+        const double t = frac - idx;
+        return lut[idx] * (1.0 - t) + lut[idx + 1] * t;
+    }
+    // src/volumetric_disk.cpp:173-214
+    double interp_2d(const double* lut, double r, double z_abs) const {
+        const double r_frac = sclamp((r - v.r_min) / (v.r_outer - v.r_min) * (v.n_r - 1), 0.0,
+                                     static_cast<double>(v.n_r - 1));
+        const int ri = std::min(static_cast<int>(r_frac), v.n_r - 2);
+        const double tr = r_frac - ri;
+        const double zm_lo = v.z_max[ri];
+        const double zm_hi = v.z_max[std::min(ri + 1, v.n_r - 1)];
+        if ((zm_lo <= 0.0 || z_abs >= zm_lo) && (zm_hi <= 0.0 || z_abs >= zm_hi)) return 0.0;
+        double lo = 0.0;
+        if (zm_lo > 0.0 && z_abs < zm_lo) {
+            const double zf = sclamp(z_abs / zm_lo * (v.n_z - 1), 0.0, static_cast<double>(v.n_z - 1));
+            const int zi = std::min(static_cast<int>(zf), v.n_z - 2);
+            const double tz = zf - zi;
+            lo = (1.0 - tz) * lut[ri * v.n_z + zi] + tz * lut[ri * v.n_z + zi + 1];
+        }
+        double hi = 0.0;
+        const int ri1 = std::min(ri + 1, v.n_r - 1);
+        if (zm_hi > 0.0 && z_abs < zm_hi) {
+            const double zf = sclamp(z_abs / zm_hi * (v.n_z - 1), 0.0, static_cast<double>(v.n_z - 1));
+            const int zi = std::min(static_cast<int>(zf), v.n_z - 2);
+            const double tz = zf - zi;
+            hi = (1.0 - tz) * lut[ri1 * v.n_z + zi] + tz * lut[ri1 * v.n_z + zi + 1];
+        }
+        return (1.0 - tr) * lo + tr * hi;
+    }
+    double scale_height(double r) const { return interp_radial(v.H, r); }
+    double z_max_at(double r) const { return interp_radial(v.z_max, r); }
+    bool inside(double r, double z) const {   // :153-160
+        if (r <= v.r_horizon || r > v.r_outer) return false;
+        return std::fabs(z) < z_max_at(r) + 1.5 * scale_height(r);
+    }
+    double taper(double r) const {   // :143-147
+        if (r >= v.r_isco) return 1.0;
+        const double dr = v.r_isco - r;
+        return std::exp(-(dr * dr) / (v.taper_w * v.taper_w));
+    }
+    // src/noise.cpp:38-115
+    double simplex(double x, double y, double z) const { return simplex_noise(v.perm, x, y, z); }
+    static double simplex_noise(const int* perm, double x, double y, double z) {
+        static const double grad3[12][3] = {{1, 1, 0}, {-1, 1, 0}, {1, -1, 0}, {-1, -1, 0},
+                                            {1, 0, 1}, {-1, 0, 1}, {1, 0, -1}, {-1, 0, -1},
+                                            {0, 1, 1}, {0, -1, 1}, {0, 1, -1}, {0, -1, -1}};
+        const double F3 = 1.0 / 3.0, G3 = 1.0 / 6.0;
+        const double s = (x + y + z) * F3;
+        const int i = static_cast<int>(std::floor(x + s)), j = static_cast<int>(std::floor(y + s)),
+                  k = static_cast<int>(std::floor(z + s));
+        const double t = (i + j + k) * G3;
+        const double X0 = i - t, Y0 = j - t, Z0 = k - t;
+        const double x0 = x - X0, y0 = y - Y0, z0 = z - Z0;
+        int i1, j1, k1, i2, j2, k2;
+        if (x0 >= y0) {
+            if (y0 >= z0) { i1 = 1; j1 = 0; k1 = 0; i2 = 1; j2 = 1; k2 = 0; }
+            else if (x0 >= z0) { i1 = 1; j1 = 0; k1 = 0; i2 = 1; j2 = 0; k2 = 1; }
+            else { i1 = 0; j1 = 0; k1 = 1; i2 = 1; j2 = 0; k2 = 1; }
+        } else {
+            if (y0 < z0) { i1 = 0; j1 = 0; k1 = 1; i2 = 0; j2 = 1; k2 = 1; }
+            else if (x0 < z0) { i1 = 0; j1 = 1; k1 = 0; i2 = 0; j2 = 1; k2 = 1; }
+            else { i1 = 0; j1 = 1; k1 = 0; i2 = 1; j2 = 1; k2 = 0; }
+        }
+        const double x1 = x0 - i1 + G3, y1 = y0 - j1 + G3, z1 = z0 - k1 + G3;
+        const double x2 = x0 - i2 + 2 * G3, y2 = y0 - j2 + 2 * G3, z2 = z0 - k2 + 2 * G3;
+        const double x3 = x0 - 1 + 3 * G3, y3 = y0 - 1 + 3 * G3, z3 = z0 - 1 + 3 * G3;
+        const int ii = i & 255, jj = j & 255, kk = k & 255;
+        auto contrib = [&](double cx, double cy, double cz, int gi) {
+            double t0 = 0.6 - cx * cx - cy * cy - cz * cz;
+            if (t0 < 0.0) return 0.0;
+            t0 *= t0;
+            const double* g = grad3[gi % 12];
+            return t0 * t0 * (g[0] * cx + g[1] * cy + g[2] * cz);
+        };
+        const double n0 = contrib(x0, y0, z0, perm[ii + perm[jj + perm[kk]]]);
+        const double n1 = contrib(x1, y1, z1, perm[ii + i1 + perm[jj + j1 + perm[kk + k1]]]);
+        const double n2 = contrib(x2, y2, z2, perm[ii + i2 + perm[jj + j2 + perm[kk + k2]]]);
+        const double n3 = contrib(x3, y3, z3, perm[ii + 1 + perm[jj + 1 + perm[kk + 1]]]);
+        return 32.0 * (n0 + n1 + n2 + n3);
+    }
+    double fbm(double x, double y, double z, int oct) const {
+        double res = 0.0, amp = 1.0, freq = 1.0;
+        for (int i = 0; i < oct; ++i) {
+            res += amp * simplex(x * freq, y * freq, z * freq);
+            amp *= 0.5;
+            freq *= 2.0;
+        }
+        return res;
+    }
+    // :226-257
+    double density(double r, double z, double phi) const {
+        if (r <= v.r_horizon || r > v.r_outer) return 0.0;
+        const double z_abs = std::fabs(z);
+        const double zm = z_max_at(r), H = scale_height(r);
+        double edge = 1.0, zl = z_abs;
+        if (z_abs >= zm) {
+            const double dz = z_abs - zm;
+            if (dz >= 1.5 * H) return 0.0;
+            edge = std::exp(-2.0 * (dz / H) * (dz / H));
+            zl = zm * 0.9999;
+        }
+        const double base = interp_radial(v.rho_mid, r) * interp_2d(v.rho_prof, r, zl) * v.rho_scale *
+                            taper(r) * edge;
+        const double nx = r * std::cos(phi) / v.noise_scale;
+        const double ny = r * std::sin(phi) / v.noise_scale;
+        const double nz = z / v.noise_scale;
+        const double n = fbm(nx, ny, nz, v.octaves);
+        return smax(0.0, base * (1.0 + v.turbulence * n));
+    }
+    // :264-273
+    double temperature(double r, double z) const {
+        if (r <= v.r_horizon || r > v.r_outer) return 0.0;
+        const double z_abs = std::fabs(z);
+        const double zm = z_max_at(r), H = scale_height(r);
+        if (z_abs >= zm + 1.5 * H) return 0.0;
+        return interp_2d(v.T_prof, r, smin(z_abs, zm * 0.9999));
+    }
+    double omega_orb(double r) const { return std::sqrt(v.M) / (r * std::sqrt(r) + v.a * std::sqrt(v.M)); }
+    // :95-133
+    void circular(double r, double& ut, double& uphi) const {
+        const double M = v.M, a = v.a, Om = omega_orb(r);
+        const double gtt = -(1.0 - 2.0 * M / r), gtp = -2.0 * M * a / r, gpp = r * r + a * a + 2.0 * M * a * a / r;
+        const double den = -(gtt + 2.0 * gtp * Om + gpp * Om * Om);
+        ut = 1.0 / std::sqrt(smax(den, 1e-30));
+        uphi = Om * ut;
+    }
+    void plunging(double r, double theta, double& ut, double& ur, double& uphi) const {
+        const double M = v.M, a = v.a, E = v.E_isco, L = v.L_isco;
+        const double Delta = r * r - 2.0 * M * r + a * a;
+        const double Sigma = r * r + a * a * std::cos(theta) * std::cos(theta);
+        ut = (E * (r * r + a * a + 2.0 * M * a * a / r) - 2.0 * M * a * L / r) / Delta;
+        uphi = (L * (1.0 - 2.0 * M / r) + 2.0 * M * a * E / r) / Delta;
+        const double EaL = E * (r * r + a * a) - a * L, LaE = L - a * E;
+        const double R = EaL * EaL - Delta * (r * r + LaE * LaE);
+        ur = -std::sqrt(smax(0.0, R)) / Sigma;
+    }
+    // src/opacity.cpp:218-346
+    static void log_cell(double lv, double lo, double hi, int n, int& idx, double& frac) {
+        double t = (lv - lo) / (hi - lo) * (n - 1);
+        t = sclamp(t, 0.0, static_cast<double>(n - 2));
+        idx = static_cast<int>(t);
+        frac = t - idx;
+    }
+    double kappa_abs(double nu, double rho, double T) const {
+        int inu, irho, iT;
+        double fnu, frho, fT;
+        log_cell(std::log10(nu), v.lnu0, v.lnu1, v.n_nu, inu, fnu);
+        log_cell(std::log10(rho), v.lrho0, v.lrho1, v.n_rho, irho, frho);
+        log_cell(std::log10(T), v.lT0, v.lT1, v.n_T, iT, fT);
+        auto idx = [&](int i, int j, int k) { return static_cast<size_t>(i) * v.n_rho * v.n_T + j * v.n_T + k; };
+        auto sl = [](double x) { return std::log(smax(x, 1e-100)); };
+        const double l00 = sl(v.kabs[idx(inu, irho, iT)]) * (1 - fT) + sl(v.kabs[idx(inu, irho, iT + 1)]) * fT;
+        const double l01 = sl(v.kabs[idx(inu, irho + 1, iT)]) * (1 - fT) + sl(v.kabs[idx(inu, irho + 1, iT + 1)]) * fT;
+        const double l10 = sl(v.kabs[idx(inu + 1, irho, iT)]) * (1 - fT) + sl(v.kabs[idx(inu + 1, irho, iT + 1)]) * fT;
+        const double l11 = sl(v.kabs[idx(inu + 1, irho + 1, iT)]) * (1 - fT) + sl(v.kabs[idx(inu + 1, irho + 1, iT + 1)]) * fT;
+        const double l0 = l00 * (1 - frho) + l01 * frho;
+        const double l1 = l10 * (1 - frho) + l11 * frho;
+        return std::exp(l0 * (1 - fnu) + l1 * fnu);
+    }
+    double kappa_es(double rho, double T) const {
+        int irho, iT;
+        double frho, fT;
+        log_cell(std::log10(rho), v.lrho0, v.lrho1, v.n_rho, irho, frho);
+        log_cell(std::log10(T), v.lT0, v.lT1, v.n_T, iT, fT);
+        const double* q = v.kes;
+        const int n = v.n_T;
+        return q[irho * n + iT] * (1 - frho) * (1 - fT) + q[irho * n + iT + 1] * (1 - frho) * fT +
+               q[(irho + 1) * n + iT] * frho * (1 - fT) + q[(irho + 1) * n + iT + 1] * frho * fT;
+    }
+};
+
+// src/opacity.cpp:209-214
+double planck_nu(double nu, double T) {
+    if (nu <= 0.0 || T <= 0.0) return 0.0;
+    const double x = kH * nu / (kKB * T);
+    if (x > 500.0) return 0.0;
+    return (2.0 * kH * nu * nu * nu / (kC * kC)) / (std::exp(x) - 1.0);
+}
+
+// Backward ray-march (src/geodesic_tracer.cpp:183-337 RGB, :605-746
+// spectral).  nb channels at frequencies nu; RGB mode exits on opacity,
+// spectral mode on tau > 10 in every bin; step control on bin `med`.
+void raymarch(const Scene& S, const Vol& VD, State& state, int nb, const double* nu, bool spectral,
+              double* J, double* T, double* tau, bool& emitted) {
+    const double M = S.p.mass, a = S.p.spin_a;
+    const double ut_obs = 1.0 / std::sqrt(1.0 - 2.0 / S.p.r_obs);
+    const int med = spectral ? nb / 2 : 1;
+    double r = state.x[1];
+    const double z_start = r * std::cos(state.x[2]);
+    const double H_start = VD.scale_height(r);
+    double ds = VD.inside(r, z_start) ? H_start / 16.0 : smin(std::fabs(z_start) / 8.0, H_start * 2.0);
+    bool been_inside = VD.inside(r, z_start);
+    int step_count = 0;
+    while (step_count < 4096) {
+        State ns;
+        rk4_step(M, a, state, ds, ns);
+        ++step_count;
+        r = ns.x[1];
+        const double theta = ns.x[2], phi = ns.x[3];
+        const double z = r * std::cos(theta);
+        if (r < VD.v.r_horizon) { state = ns; break; }
+        if (r > VD.v.r_outer) { state = ns; break; }
+        if (!spectral) {
+            if (T[0] < 1e-6 && T[1] < 1e-6 && T[2] < 1e-6) { state = ns; break; }
+        } else {
+            bool thick = true;
+            for (int c = 0; c < nb; ++c)
+                if (tau[c] <= 10.0) { thick = false; break; }
+            if (thick) { state = ns; break; }
+        }
+        const double H = VD.scale_height(r);
+        if (!VD.inside(r, z)) {
+            const double zm = VD.z_max_at(r);
+            if (been_inside && std::fabs(z) > zm + 3.0 * H) { state = ns; break; }
+            if (!been_inside) {
+                ds = smin(std::fabs(z) / 8.0, H * 2.0);
+                ds = smax(ds, H / 64.0);
+            } else {
+                ds = sclamp(H / 4.0, H / 64.0, H);
+            }
+            state = ns;
+            continue;
+        }
+        been_inside = true;
+        const double rho = VD.density(r, z, phi);
+        const double Tl = VD.temperature(r, std::fabs(z));
+        if (rho <= 0.0 || Tl <= 0.0) { state = ns; continue; }
+        double ut = 0.0, ur = 0.0, uphi = 0.0;
+        if (r >= VD.v.r_isco) VD.circular(r, ut, uphi);
+        else VD.plunging(r, theta, ut, ur, uphi);
+        const double pue = ns.p[0] * ut + ns.p[1] * ur + ns.p[3] * uphi;
+        const double puo = ns.p[0] * ut_obs;
+        const double g = pue / puo;
+        const double dsp = std::fabs(pue) * std::fabs(ds);
+        for (int c = 0; c < nb; ++c) {
+            const double nue = std::fabs(g) * nu[c];
+            const double kabs = VD.kappa_abs(nue, rho, Tl);
+            const double kes = VD.kappa_es(rho, Tl);
+            const double ktot = kabs + kes;
+            const double eps = (ktot > 0.0) ? kabs / ktot : 1.0;
+            const double dtau = ktot * rho * dsp;
+            if (spectral) tau[c] += dtau;
+            const double Bnu = planck_nu(nue, Tl);
+            const double Sv = eps * Bnu / (nue * nue * nue);
+            const double ex = std::exp(-dtau);
+            J[c] += T[c] * Sv * (1.0 - ex);
+            T[c] *= ex;
+        }
+        const double alpha_tot = (VD.kappa_abs(std::fabs(g) * nu[med], rho, Tl) + VD.kappa_es(rho, Tl)) * rho;
+        const double ds_tau = (alpha_tot > 0.0) ? 0.05 / alpha_tot : ds * 2.0;
+        const double ds_geo = 0.1 * smax(r - VD.v.r_horizon, 0.5);
+        ds = smin(ds_tau, ds_geo);
+        ds = sclamp(ds, H / 64.0, H);
+        state = ns;
+        emitted = true;
+    }
+}
+
+// Outer loop with a volumetric disk (src/geodesic_tracer.cpp:23-181 RGB,
+// :495-603 spectral).  Writes the ray record (RGB colour incl. stars) or the
+// spectrum.
+void vol_trace(const Scene& S, const Vol& VD, const State& init, int nbins, const double* nu_spec,
+               OracleRay& o, double* spectrum) {
+    const double M = S.p.mass, a = S.p.spin_a;
+    const double half_pi = kPi / 2.0;
+    const bool spectral = nbins > 0;
+    const double nu_rgb[3] = {kC / 450e-7, kC / 550e-7, kC / 650e-7};
+    const int nb = spectral ? nbins : 3;
+    const double* nu = spectral ? nu_spec : nu_rgb;
+    std::vector<double> J(nb, 0.0), T(nb, 1.0), tau(nb, 0.0);
+    State st = init, prev;
+    double dl = 0.01 * S.p.r_obs;
+    int term = 2;
+    bool emitted = false;
+    o.steps = o.attempts = o.n_cross = 0;
+    for (int step = 0; step < S.p.max_steps; ++step) {
+        const double r = st.x[1];
+        if (r < S.r_h + 0.01) { term = 0; break; }
+        if (r > S.p.r_escape) { term = 1; break; }
+        prev = st;
+        {
+            const double theta = st.x[2];
+            const double z = r * std::cos(theta);
+            if (r >= VD.v.r_horizon * 0.9 && r <= VD.v.r_outer * 1.5) {
+                const double H = VD.scale_height(r), zm = VD.z_max_at(r);
+                if (std::fabs(z) < zm + 6.0 * H) {
+                    State d;
+                    kerr_derivs(M, a, st, d);
+                    const double dz = std::fabs(std::cos(theta) * d.x[1] - r * std::sin(theta) * d.x[2]);
+                    if (dz > 1e-20) dl = smin(dl, smax(0.25 * H / dz, H / 64.0));
+                }
+            }
+        }
+        State nxt;
+        dl = adaptive_step(M, a, st, dl, S.p.tol, nxt, o.attempts);
+        st = nxt;
+        ++o.steps;
+        const bool opaque = !spectral && (T[0] < 1e-6 && T[1] < 1e-6 && T[2] < 1e-6);
+        if (!opaque) {
+            const double tp = prev.x[2], tn = st.x[2];
+            const double dp = tp - half_pi, dn = tn - half_pi;
+            const double rn = st.x[1], rp = prev.x[1];
+            const double zn = rn * std::cos(tn), zp = rp * std::cos(tp);
+            const bool crossed = (dp * dn < 0.0) && std::fabs(dp - dn) > 1e-12;
+            const bool inside_now = VD.inside(rn, zn);
+            const double zmn = VD.z_max_at(rn), Hn = VD.scale_height(rn), Hp = VD.scale_height(rp);
+            const bool near = (std::fabs(zn) < zmn + 2.0 * Hn || std::fabs(zp) < VD.z_max_at(rp) + 2.0 * Hp) &&
+                              rn >= VD.v.r_horizon && rn <= VD.v.r_outer;
+            if (crossed || inside_now || near) {
+                const double rlo = smin(rp, rn), rhi = smax(rp, rn);
+                if (rhi >= VD.v.r_horizon && rlo <= VD.v.r_outer) {
+                    State entry = prev;
+                    const double re = entry.x[1];
+                    if (re >= VD.v.r_horizon * 0.9 && re <= VD.v.r_outer * 1.5) {
+                        raymarch(S, VD, entry, nb, nu, spectral, J.data(), T.data(), tau.data(), emitted);
+                        ++o.n_cross;
+                        st = entry;
+                        continue;
+                    }
+                }
+            }
+        }
+    }
+    double col[3] = {0.0, 0.0, 0.0};
+    if (spectral) {
+        for (int c = 0; c < nb; ++c) spectrum[c] = J[c] * nu[c] * nu[c] * nu[c];
+    } else {
+        for (int c = 0; c < 3; ++c) col[c] += J[c] * nu[c] * nu[c] * nu[c];
+        if (term == 1 && S.p.stars) {
+            const double b = star_catalog().sample(st.x[2], st.x[3]);
+            for (int c = 0; c < 3; ++c) col[c] += b;
+        }
+    }
+    for (int c = 0; c < 3; ++c) o.color[c] = col[c];
+    for (int m = 0; m < 4; ++m) { o.x_final[m] = st.x[m]; o.p_final[m] = st.p[m]; }
+    o.term = term;
+    o.cls = emitted ? 2 : (term == 0 ? 0 : (term == 1 ? 1 : 3));
+    o.r_hit = o.g_hit = o.h_max = o.h_final = 0.0;
+    o.pad = 0;
 }
 
 }  // namespace
@@ -827,7 +1179,7 @@ void oracle_trace_samples(const OracleScene* sc, int mode, int spp, uint64_t see
 #endif
     for (int64_t k = 0; k < n; ++k) {
         double px, py;
-        sample_pos(S, mode, grid, nS, seed, ids[k], px, py);
+        sample_position_of(S, mode, grid, nS, seed, ids[k], px, py);
         State st;
         camera_ray(S.cam, px, py, st);
         trace(S, st, rays[k], true);
@@ -852,7 +1204,7 @@ int64_t oracle_render(const OracleScene* sc, int mode, int spp, uint64_t seed, f
         for (int s = 0; s < nS; ++s) {
             const int64_t id = pix * nS + s;
             double px, py;
-            sample_pos(S, mode, grid, nS, seed, id, px, py);
+            sample_position_of(S, mode, grid, nS, seed, id, px, py);
             State st;
             camera_ray(S.cam, px, py, st);
             OracleRay o;
@@ -868,5 +1220,35 @@ int64_t oracle_render(const OracleScene* sc, int mode, int spp, uint64_t seed, f
     }
     return total_steps;
 }
+
+void oracle_vol_trace_samples(const OracleScene* sc, const OracleVol* V, int mode, int spp, uint64_t seed,
+                              const int64_t* ids, int64_t n, int nbins, const double* nu, OracleRay* rays,
+                              double* spectra, int threads) {
+    Scene S;
+    build_scene(*sc, S);
+    const Vol VD{*V};
+    int grid;
+    const int nS = samples_per_pixel(mode, spp, grid);
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic)
+#endif
+    for (int64_t k = 0; k < n; ++k) {
+        double px, py;
+        sample_position_of(S, mode, grid, nS, seed, ids[k], px, py);
+        State st;
+        camera_ray(S.cam, px, py, st);
+        OracleRay tmp;
+        vol_trace(S, VD, st, nbins, nu, rays ? rays[k] : tmp, spectra ? spectra + k * nbins : nullptr);
+    }
+}
+
+double oracle_vol_density(const OracleVol* V, double r, double z, double phi) { return Vol{*V}.density(r, z, phi); }
+double oracle_vol_temperature(const OracleVol* V, double r, double z) { return Vol{*V}.temperature(r, z); }
+double oracle_vol_kappa_abs(const OracleVol* V, double nu, double rho, double T) {
+    return Vol{*V}.kappa_abs(nu, rho, T);
+}
+double oracle_vol_kappa_es(const OracleVol* V, double rho, double T) { return Vol{*V}.kappa_es(rho, T); }
+double oracle_simplex(const int* perm, double x, double y, double z) { return Vol::simplex_noise(perm, x, y, z); }
 
 }  // extern "C"
