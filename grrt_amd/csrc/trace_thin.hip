@@ -255,7 +255,8 @@ void trace_thin(const TraceParams* Pg) {
         const unsigned long long idle = __ballot(!live);
         const int n_idle = __popcll(idle);
         if (queue_open && (n_idle >= Q.refill_min || n_idle == 64)) {
-            const unsigned long long avail = cur_end - cur;
+            // cur never passes cur_end (see the advance below), so avail >= 0.
+            const unsigned long long avail = cur_end > cur ? cur_end - cur : 0ull;
             unsigned long long nb = ~0ull;
             if (avail < static_cast<unsigned long long>(n_idle) && have_pre) {
                 const unsigned long long got = __shfl(pre, 0);
@@ -293,8 +294,9 @@ void trace_thin(const TraceParams* Pg) {
             if (avail >= static_cast<unsigned long long>(n_idle)) {
                 cur += n_idle;
             } else if (nb != ~0ull) {
-                cur = nb + (n_idle - avail);
                 cur_end = nb + CH < nR ? nb + CH : nR;
+                const unsigned long long next = nb + (n_idle - avail);
+                cur = next < cur_end ? next : cur_end;   // the last chunk may be short
             } else {
                 cur = cur_end;
             }
