@@ -94,7 +94,7 @@ struct GRRTContext {
     bool records_valid = false;
     int tune_block = 256;
     int tune_blocks_per_cu = 2;
-    int tune_refill = 4;
+    int tune_refill = 8;
     int tune_chunk = 128;
 
     GRRTStats last{};

