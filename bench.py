@@ -71,7 +71,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--tune", default="", help="block,blocks_per_cu,refill_min,chunk")
+    ap.add_argument("--tune", default="", help="block,blocks_per_cu,refill_min,chunk[,tile]")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
@@ -85,8 +85,8 @@ def main() -> int:
     ctx = grrt.Context(params)
     ctx.set_sampling(grrt.SAMPLING_SEEDED, SPP, SEED)
     if args.tune:
-        b, bpc, rf, ch = (int(x) for x in args.tune.split(","))
-        ctx.set_tuning(b, bpc, rf, ch)
+        t = [int(x) for x in args.tune.split(",")]
+        ctx.set_tuning(t[0], t[1], t[2], t[3], t[4] if len(t) > 4 else -1)
     for _ in range(args.warmup):
         ctx.render_device()
     trace_ms, attempts, steps = [], 0, 0

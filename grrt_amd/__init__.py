@@ -77,6 +77,27 @@ class GRRTStats(C.Structure):
     ]
 
 
+class GRRTView(C.Structure):
+    _fields_ = [
+        ("spin", C.c_double), ("observer_r", C.c_double), ("observer_theta", C.c_double),
+        ("observer_phi", C.c_double), ("fov", C.c_double), ("width", C.c_int), ("height", C.c_int),
+    ]
+
+
+def views_array(views) -> "C.Array":
+    """[(spin, r, theta, phi, fov, width, height) or dict] -> GRRTView array."""
+    arr = (GRRTView * len(views))()
+    for k, v in enumerate(views):
+        if isinstance(v, dict):
+            for f, _ in GRRTView._fields_:
+                if f in v:
+                    setattr(arr[k], f, v[f])
+        else:
+            for (f, _), x in zip(GRRTView._fields_, v):
+                setattr(arr[k], f, x)
+    return arr
+
+
 PROGRESS_FN = C.CFUNCTYPE(None, C.c_float, C.c_void_p)
 
 # Every symbol the C headers declare: (name, restype, argtypes).
@@ -108,7 +129,10 @@ API = [
     ("grrt_ext_get_classes", C.c_int, [_P, _P, C.c_int64]),
     ("grrt_ext_last_stats", C.c_int, [_P, C.POINTER(GRRTStats)]),
     ("grrt_ext_render_device", C.c_int, [_P, _P]),
-    ("grrt_ext_set_tuning", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int]),
+    ("grrt_ext_set_tuning", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+    ("grrt_ext_render_batch", C.c_int, [_P, C.POINTER(GRRTView), C.c_int, _P, C.c_int]),
+    ("grrt_ext_autotune", C.c_int, [_P, C.POINTER(GRRTView), C.c_int, C.POINTER(C.c_int),
+                                    C.POINTER(C.c_double)]),
     ("grrt_ext_set_precision", C.c_int, [_P, C.c_int]),
     ("grrt_ext_render_spectral_f32", C.c_int, [_P, C.POINTER(C.c_float), C.c_int, C.c_int]),
     ("grrt_ext_vol_build", _P, [C.POINTER(GRRTParams)]),
@@ -260,9 +284,37 @@ class Context:
     def set_records(self, enable: bool = True):
         self._check(self._L.grrt_ext_set_records(self._h, 1 if enable else 0), "set_records")
 
-    def set_tuning(self, block=0, blocks_per_cu=0, refill_min=0, chunk=0):
-        self._check(self._L.grrt_ext_set_tuning(self._h, block, blocks_per_cu, refill_min, chunk),
+    def set_tuning(self, block=0, blocks_per_cu=0, refill_min=0, chunk=0, tile=-1):
+        self._check(self._L.grrt_ext_set_tuning(self._h, block, blocks_per_cu, refill_min, chunk, tile),
                     "set_tuning")
+
+    def render_batch(self, views) -> list[np.ndarray]:
+        """Render several views in one persistent launch (grrt_ext_render_batch);
+        returns one (height, width, 4) float32 frame per view."""
+        arr = views_array(views)
+        npx = sum(v.width * v.height for v in arr)
+        out = np.zeros(4 * npx, np.float32)
+        self._check(self._L.grrt_ext_render_batch(self._h, arr, len(arr), out.ctypes.data, 0),
+                    "grrt_ext_render_batch")
+        frames, o = [], 0
+        for v in arr:
+            n = v.width * v.height * 4
+            frames.append(out[o:o + n].reshape(v.height, v.width, 4))
+            o += n
+        return frames
+
+    def autotune(self, views=None) -> tuple[dict, float]:
+        """Time launch shapes on the frame (views None) or the batch and keep
+        the fastest; returns ({tile, blocks_per_cu, refill_min, chunk}, ms)."""
+        best = (C.c_int * 4)()
+        ms = C.c_double()
+        if views is None:
+            rc = self._L.grrt_ext_autotune(self._h, None, 0, best, C.byref(ms))
+        else:
+            arr = views_array(views)
+            rc = self._L.grrt_ext_autotune(self._h, arr, len(arr), best, C.byref(ms))
+        self._check(rc, "grrt_ext_autotune")
+        return dict(zip(("tile", "blocks_per_cu", "refill_min", "chunk"), list(best))), ms.value
 
     def render(self, progress=None) -> np.ndarray:
         fb = np.zeros((self.params.height, self.params.width, 4), np.float32)

@@ -153,6 +153,55 @@ void build_scene(GRRTContext* ctx) {
     check(hipStreamSynchronize(ctx->stream), "upload tables");
 }
 
+int samples_per_pixel(int mode, int spp, int& grid);
+
+// One view of the launch: metric, camera, window and sample placement, disk.
+void fill_view(const GRRTContext* ctx, grrt_amd::ViewParams& V, double a, double r_obs, double th_obs,
+               double ph_obs, double fov, int W, int H, const grrt_amd::CameraFrame& cam,
+               const double* flux, double flux_max, double r_in, int x0, int y0, int tw, int th, int tile) {
+    const grrt_amd::Resolved& r = ctx->res;
+    V = grrt_amd::ViewParams{};
+    V.M = r.M;
+    V.a = a;
+    V.r_term = grrt_amd::kerr_horizon(r.M, a) + 0.01;
+    V.W = W;
+    V.H = H;
+    V.r_obs = r_obs;
+    V.th_obs = th_obs;
+    V.ph_obs = ph_obs;
+    V.fov = fov;
+    V.e0t = cam.e0[0];
+    for (int m = 0; m < 4; ++m) {
+        V.e1[m] = cam.e1[m];
+        V.e2[m] = cam.e2[m];
+        V.e3[m] = cam.e3[m];
+    }
+    V.g00 = cam.g[0][0];
+    V.g03 = cam.g[0][3];
+    V.g11 = cam.g[1][1];
+    V.g22 = cam.g[2][2];
+    V.g33 = cam.g[3][3];
+    int grid = 1;
+    V.nS = samples_per_pixel(ctx->sampling_mode, ctx->sampling_spp, grid);
+    V.mode = ctx->sampling_mode;
+    V.grid = grid;
+    V.cell = grid > 0 ? 1.0 / grid : 1.0;
+    V.seed = ctx->sampling_seed;
+    V.x0 = x0;
+    V.y0 = y0;
+    V.tw = tw;
+    V.th = th;
+    V.tile = tile;
+    V.disk = r.disk;
+    V.r_in = r_in;
+    V.r_out = r.r_out;
+    V.t_peak = r.t_disk;
+    V.flux_max = flux_max;
+    V.ut_obs = 1.0 / std::sqrt(1.0 - 2.0 * r.M / r_obs);
+    V.flux_lut = flux;
+    V.n_rays = static_cast<long long>(tw) * th * V.nS;
+}
+
 int samples_per_pixel(int mode, int spp, int& grid) {
     if (mode == GRRT_SAMPLING_CENTER) { grid = 1; return 1; }
     if (mode == GRRT_SAMPLING_HASH) {
@@ -187,46 +236,13 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
     check(ctx->params_dev.ensure(1), "hipMalloc(params)");
 
     TraceParams P{};
-    P.M = r.M;
-    P.a = r.a;
-    P.r_term = r.r_h + 0.01;
+    fill_view(ctx, P.view, r.a, r.r_obs, r.th_obs, r.ph_obs, r.fov, r.W, r.H, ctx->cam, ctx->flux.ptr,
+              ctx->flux_max, r.r_in, x0, y0, tw, th, spectral || vol ? 0 : ctx->tune_tile);
     P.r_escape = 1000.0;
     P.tol = r.tol;
     P.max_steps = r.max_steps;
-    P.W = r.W;
-    P.H = r.H;
-    P.r_obs = r.r_obs;
-    P.th_obs = r.th_obs;
-    P.ph_obs = r.ph_obs;
-    P.fov = r.fov;
-    P.e0t = ctx->cam.e0[0];
-    for (int m = 0; m < 4; ++m) {
-        P.e1[m] = ctx->cam.e1[m];
-        P.e2[m] = ctx->cam.e2[m];
-        P.e3[m] = ctx->cam.e3[m];
-    }
-    P.g00 = ctx->cam.g[0][0];
-    P.g03 = ctx->cam.g[0][3];
-    P.g11 = ctx->cam.g[1][1];
-    P.g22 = ctx->cam.g[2][2];
-    P.g33 = ctx->cam.g[3][3];
-    P.mode = ctx->sampling_mode;
-    P.grid = grid;
-    P.nS = nS;
-    P.cell = grid > 0 ? 1.0 / grid : 1.0;
-    P.seed = ctx->sampling_seed;
-    P.x0 = x0;
-    P.y0 = y0;
-    P.tw = tw;
-    P.th = th;
-    P.n_rays = n_rays;
-    P.disk = r.disk;
-    P.r_in = r.r_in;
-    P.r_out = r.r_out;
-    P.t_peak = r.t_disk;
-    P.flux_max = ctx->flux_max;
-    P.ut_obs = 1.0 / std::sqrt(1.0 - 2.0 * r.M / r.r_obs);
-    P.flux_lut = ctx->flux.ptr;
+    P.n_rays = view_queue_length(P.view);
+    P.view.n_rays = n_rays;
     P.spec_lut = ctx->spec.ptr;
     P.stars = r.stars;
     P.star_start = ctx->star_start.ptr;
@@ -265,17 +281,17 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
     *ctx->params_host = P;
     check(hipMemcpyAsync(ctx->params_dev.ptr, ctx->params_host, sizeof(TraceParams), hipMemcpyHostToDevice,
                          ctx->stream), "upload params");
-    check(launch_reset(P.counters, ctx->stream), "reset kernel");
+    check(launch_reset(P.counters, P.sample_cls, n_rays, ctx->stream), "reset kernel");
     check(hipEventRecord(ctx->ev[0], ctx->stream), "hipEventRecord");
     if (vol) check(launch_trace_vol(ctx->params_dev.ptr, n_rays, spectral, ctx->mixed_precision, ctx->stream),
                    "volumetric trace launch");
-    else check(launch_trace_thin(ctx->params_dev.ptr, blocks, threads, ctx->stream), "trace kernel launch");
+    else check(launch_trace_thin(ctx->params_dev.ptr, false, blocks, threads, ctx->stream), "trace kernel launch");
     check(hipEventRecord(ctx->ev[1], ctx->stream), "hipEventRecord");
     if (spectral)
         check(launch_finalize_spectral(P.sample_spec, d_out, npix, nS, nb, kind == Output::kSpectralF32, ctx->stream),
               "finalize launch");
     else
-        check(launch_finalize(P.sample_rgb, static_cast<float*>(d_out), npix, nS, P.counters, ctx->stream),
+        check(launch_finalize(P.sample_rgb, P.sample_cls, static_cast<float*>(d_out), npix, nS, ctx->stream),
               "finalize launch");
     check(hipEventRecord(ctx->ev[2], ctx->stream), "hipEventRecord");
 
@@ -291,7 +307,7 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
     float t_trace = 0.0f, t_fin = 0.0f;
     check(hipEventElapsedTime(&t_trace, ctx->ev[0], ctx->ev[1]), "event time");
     check(hipEventElapsedTime(&t_fin, ctx->ev[1], ctx->ev[2]), "event time");
-    const bool cancelled = c.queue_head < static_cast<unsigned long long>(n_rays);
+    const bool cancelled = c.rays_done < static_cast<unsigned long long>(n_rays);
     GRRTStats& s = ctx->last;
     s = GRRTStats{};
     s.trace_ms = t_trace;
@@ -305,6 +321,7 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
     s.cancelled = cancelled ? 1 : 0;
     s.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     ctx->last_rays = n_rays;
+    ctx->last_batch = false;
     ctx->records_valid = ctx->keep_records && !cancelled;
     ctx->progress.store(1.0f);
     if (progress) progress(1.0f, ud);
@@ -437,6 +454,28 @@ void grrt_destroy(GRRTContext* ctx) {
     ctx->frame.release();
     ctx->counters.release();
     ctx->params_dev.release();
+    if (ctx->batch_exec) (void)hipGraphExecDestroy(ctx->batch_exec);
+    if (ctx->batch_graph) (void)hipGraphDestroy(ctx->batch_graph);
+    ctx->batch_params.release();
+    ctx->batch_tp.release();
+    ctx->batch_rgb.release();
+    ctx->batch_cls.release();
+    ctx->batch_counters.release();
+    ctx->batch_flux.release();
+    ctx->batch_out.release();
+    ctx->vol_H.release();
+    ctx->vol_zmax.release();
+    ctx->vol_rho_mid.release();
+    ctx->vol_rho_prof.release();
+    ctx->vol_T_prof.release();
+    ctx->vol_lnk.release();
+    ctx->vol_kes.release();
+    ctx->vol_perm.release();
+    ctx->nu_obs.release();
+    ctx->lognu_obs.release();
+    ctx->rt_scratch.release();
+    ctx->sample_spec.release();
+    ctx->spec_out.release();
     if (ctx->params_host) (void)hipHostFree(ctx->params_host);
     if (ctx->cancel_flag) (void)hipHostFree(const_cast<int*>(ctx->cancel_flag));
     for (auto& e : ctx->ev)
@@ -449,6 +488,7 @@ void grrt_update_params(GRRTContext* ctx, const GRRTParams* params) {
     if (!ctx || !params) return;
     guarded(ctx, [&] {
         ctx->params = *params;
+        ctx->batch_views.clear();      // tables may move: re-record the batch
         build_scene(ctx);
         return 0;
     });
@@ -577,6 +617,7 @@ int grrt_ext_set_sampling(GRRTContext* ctx, int mode, int spp, uint64_t seed) {
         ctx->sampling_mode = mode;
         ctx->sampling_spp = std::max(spp, 1);
         ctx->sampling_seed = seed;
+        ctx->batch_views.clear();      // sample placement is recorded with the batch
         return 0;
     });
 }
@@ -591,6 +632,7 @@ int grrt_ext_set_records(GRRTContext* ctx, int enable) {
 
 int64_t grrt_ext_num_samples(const GRRTContext* ctx) {
     if (!ctx) return -1;
+    if (ctx->last_batch) return ctx->last_rays;
     int grid = 1;
     const int nS = samples_per_pixel(ctx->sampling_mode, ctx->sampling_spp, grid);
     return static_cast<int64_t>(ctx->res.W) * ctx->res.H * nS;
@@ -609,8 +651,8 @@ int grrt_ext_get_records(GRRTContext* ctx, GRRTRayRecord* out, int64_t n) {
 int grrt_ext_get_classes(GRRTContext* ctx, uint8_t* out, int64_t n) {
     return guarded(ctx, [&] {
         if (!out || n < 0 || n > ctx->last_rays) throw HipFailure("class count out of range");
-        check(hipMemcpy(out, ctx->sample_cls.ptr, static_cast<size_t>(n), hipMemcpyDeviceToHost),
-              "copy classes");
+        const uint8_t* src = ctx->last_batch ? ctx->batch_cls.ptr : ctx->sample_cls.ptr;
+        check(hipMemcpy(out, src, static_cast<size_t>(n), hipMemcpyDeviceToHost), "copy classes");
         return 0;
     });
 }
@@ -633,7 +675,7 @@ int grrt_ext_render_device(GRRTContext* ctx, float* d_rgba) {
     });
 }
 
-int grrt_ext_set_tuning(GRRTContext* ctx, int block, int blocks_per_cu, int refill_min, int chunk) {
+int grrt_ext_set_tuning(GRRTContext* ctx, int block, int blocks_per_cu, int refill_min, int chunk, int tile) {
     return guarded(ctx, [&] {
         if (block > 0) {
             if (block % 64 != 0 || block > 256) throw HipFailure("block must be a multiple of 64, at most 256");
@@ -644,6 +686,10 @@ int grrt_ext_set_tuning(GRRTContext* ctx, int block, int blocks_per_cu, int refi
         if (chunk > 0) {
             if (chunk < 64) throw HipFailure("chunk must be at least 64 rays");
             ctx->tune_chunk = chunk;
+        }
+        if (tile >= 0) {
+            if (tile == 1 || tile > 32) throw HipFailure("tile must be 0 (row order) or 2..32 pixels");
+            ctx->tune_tile = tile;
         }
         return 0;
     });
@@ -744,6 +790,198 @@ int grrt_ext_opacity(double nu, double rho, double T, double out[6]) {
     out[4] = grrt_amd::alpha_bf_point(nu, T, s);
     out[5] = grrt_amd::planck_nu_host(nu, T);
     return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// (Re)build the batch's per-view tables and parameter blocks, and record
+// the launch sequence as a graph.
+void prepare_batch(GRRTContext* ctx, const GRRTView* views, int n) {
+    using namespace grrt_amd;
+    const Resolved& r = ctx->res;
+    if (r.volumetric) throw HipFailure("batched rendering covers the thin disk only");
+    std::vector<ViewParams> vp(n);
+    std::vector<double> flux(static_cast<size_t>(n) * kFluxN, 0.0);
+    check(ctx->batch_flux.ensure(flux.size()), "hipMalloc(batch flux)");
+    long long q = 0, rays = 0, pix = 0;
+    for (int v = 0; v < n; ++v) {
+        const GRRTView& w = views[v];
+        if (w.width <= 0 || w.height <= 0) throw HipFailure("batch view with an empty frame");
+        if (!(std::abs(w.spin) < 1.0)) throw HipFailure("batch view spin must satisfy |a/M| < 1");
+        const double a = w.spin * r.M;
+        const double r_obs = w.observer_r > 0.0 ? w.observer_r : r.r_obs;
+        const double th = w.observer_theta > 0.0 ? w.observer_theta : r.th_obs;
+        const double fov = w.fov > 0.0 ? w.fov : r.fov;
+        const double r_in = ctx->params.disk_inner > 0.0 ? ctx->params.disk_inner : kerr_isco(r.M, a);
+        double fmax = 0.0;
+        std::vector<double> f;
+        if (r.disk) {
+            fmax = build_flux_table(r.M, a, r_in, r.r_out, f);
+            std::copy(f.begin(), f.end(), flux.begin() + static_cast<size_t>(v) * kFluxN);
+        }
+        const CameraFrame cam = build_camera(r.M, a, r_obs, th, w.observer_phi);
+        fill_view(ctx, vp[v], a, r_obs, th, w.observer_phi, fov, w.width, w.height, cam,
+                  ctx->batch_flux.ptr + static_cast<size_t>(v) * kFluxN, fmax, r_in, 0, 0, w.width, w.height,
+                  ctx->tune_tile);
+        vp[v].q_offset = q;
+        vp[v].ray_offset = rays;
+        vp[v].pix_offset = pix;
+        q += view_queue_length(vp[v]);
+        rays += vp[v].n_rays;
+        pix += static_cast<long long>(w.width) * w.height;
+    }
+    upload(ctx->batch_flux, flux, ctx->stream);
+    upload(ctx->batch_params, vp, ctx->stream);
+    const int nS = vp[0].nS;
+    check(ctx->batch_rgb.ensure(static_cast<size_t>(3 * rays)), "hipMalloc(batch samples)");
+    check(ctx->batch_cls.ensure(static_cast<size_t>(rays)), "hipMalloc(batch classes)");
+    check(ctx->batch_out.ensure(static_cast<size_t>(4 * pix)), "hipMalloc(batch frames)");
+    check(ctx->batch_counters.ensure(1), "hipMalloc(batch counters)");
+    check(ctx->batch_tp.ensure(1), "hipMalloc(batch params)");
+    TraceParams P{};
+    P.view = vp[0];
+    P.views = ctx->batch_params.ptr;
+    P.n_views = n;
+    P.r_escape = 1000.0;
+    P.tol = r.tol;
+    P.max_steps = r.max_steps;
+    P.n_rays = q;
+    P.spec_lut = ctx->spec.ptr;
+    P.stars = r.stars;
+    P.star_start = ctx->star_start.ptr;
+    P.star_tpb = ctx->star_tpb.ptr;
+    P.sample_rgb = ctx->batch_rgb.ptr;
+    P.sample_cls = ctx->batch_cls.ptr;
+    P.records = nullptr;
+    P.counters = ctx->batch_counters.ptr;
+    P.cancel = nullptr;
+    P.refill_min = std::clamp(ctx->tune_refill, 1, 64);
+    P.chunk = std::max(ctx->tune_chunk, 64);
+    check(hipMemcpy(ctx->batch_tp.ptr, &P, sizeof(TraceParams), hipMemcpyHostToDevice), "hipMemcpy(batch params)");
+    // Record: reset, the persistent trace over all views, finalize.
+    if (ctx->batch_exec) (void)hipGraphExecDestroy(ctx->batch_exec);
+    if (ctx->batch_graph) (void)hipGraphDestroy(ctx->batch_graph);
+    ctx->batch_exec = nullptr;
+    ctx->batch_graph = nullptr;
+    ctx->batch_views.clear();
+    check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
+    hipError_t e = launch_reset(P.counters, P.sample_cls, rays, ctx->stream);
+    if (e == hipSuccess)
+        e = launch_trace_thin(ctx->batch_tp.ptr, true, std::max(1, ctx->num_cus * ctx->tune_blocks_per_cu),
+                              ctx->tune_block, ctx->stream);
+    if (e == hipSuccess) e = launch_finalize(P.sample_rgb, P.sample_cls, ctx->batch_out.ptr, pix, nS, ctx->stream);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+    if (e != hipSuccess && g) (void)hipGraphDestroy(g);
+    check(e, "record batch launch");
+    check(e2, "end capture");
+    ctx->batch_graph = g;
+    check(hipGraphInstantiate(&ctx->batch_exec, g, nullptr, nullptr, 0), "instantiate batch graph");
+    ctx->batch_views.assign(views, views + n);
+    ctx->batch_rays = rays;
+    ctx->batch_pixels = pix;
+    ctx->batch_tuning[0] = ctx->tune_tile;
+    ctx->batch_tuning[1] = ctx->tune_blocks_per_cu;
+    ctx->batch_tuning[2] = ctx->tune_refill;
+    ctx->batch_tuning[3] = ctx->tune_chunk;
+}
+
+bool same_batch(const GRRTContext* ctx, const GRRTView* views, int n) {
+    if (static_cast<int>(ctx->batch_views.size()) != n || !ctx->batch_exec) return false;
+    if (ctx->batch_tuning[0] != ctx->tune_tile || ctx->batch_tuning[1] != ctx->tune_blocks_per_cu ||
+        ctx->batch_tuning[2] != ctx->tune_refill || ctx->batch_tuning[3] != ctx->tune_chunk)
+        return false;
+    return std::memcmp(ctx->batch_views.data(), views, sizeof(GRRTView) * n) == 0;
+}
+
+// Replay the recorded batch; fills ctx->last.
+int run_batch(GRRTContext* ctx) {
+    using namespace grrt_amd;
+    const auto t0 = std::chrono::steady_clock::now();
+    check(hipEventRecord(ctx->ev[0], ctx->stream), "hipEventRecord");
+    check(hipGraphLaunch(ctx->batch_exec, ctx->stream), "graph launch");
+    check(hipEventRecord(ctx->ev[1], ctx->stream), "hipEventRecord");
+    check(hipEventSynchronize(ctx->ev[1]), "batch render");
+    DevCounters c{};
+    check(hipMemcpy(&c, ctx->batch_counters.ptr, sizeof(c), hipMemcpyDeviceToHost), "read counters");
+    float ms = 0.0f;
+    check(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]), "event time");
+    GRRTStats& s = ctx->last;
+    s = GRRTStats{};
+    s.trace_ms = ms;
+    s.rays = static_cast<int64_t>(c.rays_done);
+    s.steps = static_cast<int64_t>(c.steps);
+    s.attempts = static_cast<int64_t>(c.attempts);
+    s.deriv_evals = static_cast<int64_t>(c.rays_done + 6 * c.attempts);
+    s.algo_flops = flops_per_attempt() * static_cast<double>(c.attempts);
+    s.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    ctx->last_rays = ctx->batch_rays;
+    ctx->records_valid = false;
+    ctx->last_batch = true;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int grrt_ext_render_batch(GRRTContext* ctx, const GRRTView* views, int n, float* out, int device_out) {
+    return guarded(ctx, [&] {
+        if (!views || n <= 0 || !out) throw HipFailure("grrt_ext_render_batch: empty batch or null output");
+        if (!same_batch(ctx, views, n)) prepare_batch(ctx, views, n);
+        run_batch(ctx);
+        const size_t bytes = static_cast<size_t>(4 * ctx->batch_pixels) * sizeof(float);
+        check(hipMemcpy(out, ctx->batch_out.ptr, bytes, device_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost),
+              "copy batch frames");
+        return 0;
+    });
+}
+
+int grrt_ext_autotune(GRRTContext* ctx, const GRRTView* views, int n, int best[4], double* best_ms) {
+    return guarded(ctx, [&] {
+        const int tiles[] = {0, 8};
+        const int bpcs[] = {1, 2};
+        const int refills[] = {2, 4, 8, 16};
+        const int chunks[] = {64, 128};
+        double best_t = 1e300;
+        int bt[4] = {ctx->tune_tile, ctx->tune_blocks_per_cu, ctx->tune_refill, ctx->tune_chunk};
+        const auto& r = ctx->res;
+        if (r.volumetric) throw HipFailure("autotune covers the thin-disk tracer");
+        if (!views) check(ctx->frame.ensure(4 * static_cast<size_t>(r.W) * r.H), "hipMalloc(frame)");
+        for (int tile : tiles)
+            for (int bpc : bpcs)
+                for (int rf : refills)
+                    for (int ch : chunks) {
+                        ctx->tune_tile = tile;
+                        ctx->tune_blocks_per_cu = bpc;
+                        ctx->tune_refill = rf;
+                        ctx->tune_chunk = ch;
+                        double t = 0.0;
+                        if (views) {
+                            prepare_batch(ctx, views, n);
+                            run_batch(ctx);      // warm
+                            run_batch(ctx);
+                            t = ctx->last.trace_ms;
+                        } else {
+                            render_window(ctx, 0, 0, r.W, r.H, ctx->frame.ptr, nullptr, nullptr);
+                            t = ctx->last.trace_ms;
+                        }
+                        if (t < best_t) {
+                            best_t = t;
+                            bt[0] = tile; bt[1] = bpc; bt[2] = rf; bt[3] = ch;
+                        }
+                    }
+        ctx->tune_tile = bt[0];
+        ctx->tune_blocks_per_cu = bt[1];
+        ctx->tune_refill = bt[2];
+        ctx->tune_chunk = bt[3];
+        if (views) prepare_batch(ctx, views, n);
+        if (best) for (int i = 0; i < 4; ++i) best[i] = bt[i];
+        if (best_ms) *best_ms = best_t;
+        return 0;
+    });
 }
 
 }  // extern "C"

@@ -36,13 +36,13 @@ struct DevBuf {
     }
 };
 
-hipError_t launch_reset(DevCounters* c, hipStream_t st);
+hipError_t launch_reset(DevCounters* c, uint8_t* cls, long long n_rays, hipStream_t st);
 hipError_t launch_trace_vol(const TraceParams* P, long long n_rays, bool spectral, bool fp32_rt, hipStream_t st);
 hipError_t launch_finalize_spectral(const double* spec, void* out, long long npix, int nS, int nb, bool f32,
                                     hipStream_t st);
-hipError_t launch_trace_thin(const TraceParams* P, int blocks, int threads, hipStream_t st);
-hipError_t launch_finalize(const double* rgb, float* out, long long npix, int nS,
-                           const DevCounters* cnt, hipStream_t st);
+hipError_t launch_trace_thin(const TraceParams* P, bool batch, int blocks, int threads, hipStream_t st);
+hipError_t launch_finalize(const double* rgb, const uint8_t* cls, float* out, long long npix, int nS,
+                           hipStream_t st);
 double flops_per_attempt();
 
 }  // namespace grrt_amd
@@ -96,6 +96,25 @@ struct GRRTContext {
     int tune_blocks_per_cu = 2;
     int tune_refill = 8;
     int tune_chunk = 128;
+    int tune_tile = 0;
+
+    // Batched views: per-view tables and parameter blocks, and the captured
+    // launch sequence, reused while the same views are rendered again.
+    std::vector<GRRTView> batch_views;
+    grrt_amd::DevBuf<grrt_amd::ViewParams> batch_params;
+    grrt_amd::DevBuf<double> batch_flux;
+    grrt_amd::DevBuf<float> batch_out;
+    // The batch owns its launch block, samples and counters, so renders of
+    // the context's own frame between replays leave the recorded graph valid.
+    grrt_amd::DevBuf<grrt_amd::TraceParams> batch_tp;
+    grrt_amd::DevBuf<double> batch_rgb;
+    grrt_amd::DevBuf<uint8_t> batch_cls;
+    grrt_amd::DevBuf<grrt_amd::DevCounters> batch_counters;
+    bool last_batch = false;            // the last render was a batch (classes come from batch_cls)
+    hipGraph_t batch_graph = nullptr;
+    hipGraphExec_t batch_exec = nullptr;
+    long long batch_rays = 0, batch_pixels = 0;
+    int batch_tuning[4] = {0, 0, 0, 0};
 
     GRRTStats last{};
     long long last_rays = 0;

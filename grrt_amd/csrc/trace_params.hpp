@@ -1,6 +1,8 @@
 // Launch parameters of the thin-disk trace kernel, shared by host and device.
 // Passed by value as the kernel argument (well under the 4 KiB limit).
 #pragma once
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 #include "grrt_ext.h"
@@ -38,13 +40,12 @@ struct VolDev {
     const int* perm;                     // 512 entries
 };
 
-struct TraceParams {
-    // Kerr metric in Boyer–Lindquist coordinates.
+// Everything that belongs to one camera view: the metric (spin varies per
+// view in a batch), the camera, the pixel window and sample placement, and
+// the thin disk (its inner edge follows the view's ISCO).
+struct ViewParams {
     double M, a;
     double r_term;       // horizon radius + 0.01 (src/geodesic_tracer.h:64)
-    double r_escape;     // 1000 M (src/api.cpp:132)
-    double tol;
-    int max_steps;
 
     // Camera: static observer tetrad and the covariant metric there.
     int W, H;
@@ -58,12 +59,43 @@ struct TraceParams {
     double cell;
     uint64_t seed;
     int x0, y0, tw, th;                 // pixel window (full frame or tile)
-    long long n_rays;
+    int tile;                           // queue order: 0 row-major, else tile x tile pixel blocks
 
     // Thin disk (src/accretion_disk.cpp).
     int disk;
     double r_in, r_out, t_peak, flux_max, ut_obs;
     const double* flux_lut;             // kFluxN
+
+    // Place of this view in a launch: its first queue slot (queue order may
+    // pad to whole tiles), its first sample in the pixel-major sample
+    // buffers, its first output pixel, and its sample count.
+    long long q_offset, ray_offset, pix_offset, n_rays;
+};
+
+// Rays of one view in queue order: row-major samples (tile == 0) or
+// tile x tile pixel blocks, each block's samples together, so a wavefront's
+// lanes trace neighbouring rays.  The tiled order pads the window to whole
+// blocks; padded slots decode as invalid.
+__host__ __device__ inline long long view_queue_length(const ViewParams& V) {
+    const long long px = static_cast<long long>(V.tw) * V.th;
+    if (V.tile <= 0) return px * V.nS;
+    const long long nx = (V.tw + V.tile - 1) / V.tile, ny = (V.th + V.tile - 1) / V.tile;
+    return nx * ny * V.tile * V.tile * V.nS;
+}
+
+struct TraceParams {
+    ViewParams view;                    // the single view of a frame, tile or spectral render
+
+    // Integrator and termination shared by all views.
+    double r_escape;     // 1000 M (src/api.cpp:132)
+    double tol;
+    int max_steps;
+    long long n_rays;    // rays in the launch (all views of a batch)
+
+    // Batch of views (trace_batch): n_views entries sorted by ray_offset.
+    const ViewParams* views;
+    int n_views;
+
     const double* spec_lut;             // kSpecN * 3
 
     // Star background (src/celestial_sphere.cpp), CSR over the 180x360 grid.

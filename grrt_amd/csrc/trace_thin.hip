@@ -20,6 +20,8 @@
 // which is the same evaluation the reference repeats.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device_math.hpp"
 #include "kerr_rhs.hpp"
 #include "camera_device.hpp"
@@ -71,9 +73,9 @@ __device__ __forceinline__ double hamiltonian_from_rhs(const Ray& R, double dt) 
 
 // Ray start: camera covector (src/camera.cpp:69-96) and the first
 // right-hand side.
-__device__ __forceinline__ void init_ray(const TraceParams& P, const KerrK& K, long long id, Ray& R) {
-    camera_ray(P, id, R.P0, R.P3, R.y);
-    R.dl = 0.01 * P.r_obs;
+__device__ __forceinline__ void init_ray(const ViewParams& V, const SampleId& d, const KerrK& K, Ray& R) {
+    camera_ray(V, d, R.P0, R.P3, R.y);
+    R.dl = 0.01 * V.r_obs;
     R.col[0] = R.col[1] = R.col[2] = 0.0;
     R.r_hit = R.g_hit = 0.0;
     R.steps = R.attempts = R.retry = R.n_cross = 0;
@@ -84,23 +86,23 @@ __device__ __forceinline__ void init_ray(const TraceParams& P, const KerrK& K, l
 }
 
 // Thin-disk emission at an in-range crossing (src/accretion_disk.cpp:96-147).
-__device__ __forceinline__ void disk_emission(const TraceParams& P, const double* s_flux,
+__device__ __forceinline__ void disk_emission(const ViewParams& V, const double* flux,
                                               const double* s_spec, double rc, double P0,
                                               double P3, Ray& R) {
     double F = 0.0;
-    if (!(rc <= P.r_in || rc >= P.r_out || P.flux_max <= 0.0)) {
-        const double frac = (rc - P.r_in) / (P.r_out - P.r_in) * (kFluxN - 1);
+    if (!(rc <= V.r_in || rc >= V.r_out || V.flux_max <= 0.0)) {
+        const double frac = (rc - V.r_in) / (V.r_out - V.r_in) * (kFluxN - 1);
         const int idx = static_cast<int>(frac);
         const double t = frac - idx;
-        F = (idx >= kFluxN - 1) ? s_flux[kFluxN - 1] : s_flux[idx] * (1.0 - t) + s_flux[idx + 1] * t;
+        F = (idx >= kFluxN - 1) ? flux[kFluxN - 1] : flux[idx] * (1.0 - t) + flux[idx + 1] * t;
     }
     // Redshift g = (p.u)_emit / (p.u)_obs for a circular emitter and the
     // static observer (src/accretion_disk.cpp:113-130).
-    const double w = sqrt(P.M / (rc * rc * rc));
-    const double aw = P.a * w;
-    const double ut_e = 1.0 / sqrt(1.0 - 3.0 * P.M / rc + 2.0 * aw);
-    const double up_e = (w / (1.0 + P.a * w)) * ut_e;
-    const double pu_obs = P0 * P.ut_obs;
+    const double w = sqrt(V.M / (rc * rc * rc));
+    const double aw = V.a * w;
+    const double ut_e = 1.0 / sqrt(1.0 - 3.0 * V.M / rc + 2.0 * aw);
+    const double up_e = (w / (1.0 + V.a * w)) * ut_e;
+    const double pu_obs = P0 * V.ut_obs;
     const double pu_e = P0 * ut_e + P3 * up_e;
     const double g = (__builtin_fabs(pu_obs) < 1e-30) ? 1.0 : pu_e / pu_obs;
     if (R.n_cross == 0) {
@@ -108,8 +110,8 @@ __device__ __forceinline__ void disk_emission(const TraceParams& P, const double
         R.g_hit = g;
     }
     ++R.n_cross;
-    if (F <= 0.0 || P.flux_max <= 0.0) return;
-    const double T = P.t_peak * sqrt(sqrt(F / P.flux_max));
+    if (F <= 0.0 || V.flux_max <= 0.0) return;
+    const double T = V.t_peak * sqrt(sqrt(F / V.flux_max));
     if (T <= 0.0) return;
     const double Tobs = g * T;
     if (!(Tobs >= 100.0)) return;       // also drops a NaN temperature
@@ -131,7 +133,7 @@ __device__ __forceinline__ void disk_emission(const TraceParams& P, const double
     }
 }
 
-__device__ __forceinline__ void finish_ray(const TraceParams& P, long long id, Ray& R, int term) {
+__device__ __forceinline__ void finish_ray(const TraceParams& P, long long oid, Ray& R, int term) {
     if (term == 1 && P.stars) {
         const double b = star_sample(P, R.y.v[1], R.y.v[2]);
         R.col[0] += b;
@@ -141,12 +143,12 @@ __device__ __forceinline__ void finish_ray(const TraceParams& P, long long id, R
     const int cls = (R.n_cross > 0) ? GRRT_CLASS_DISK
                                     : (term == 0 ? GRRT_CLASS_CAPTURED
                                                  : (term == 1 ? GRRT_CLASS_ESCAPED : GRRT_CLASS_MAXSTEPS));
-    P.sample_rgb[3 * id + 0] = R.col[0];
-    P.sample_rgb[3 * id + 1] = R.col[1];
-    P.sample_rgb[3 * id + 2] = R.col[2];
-    P.sample_cls[id] = static_cast<uint8_t>(cls);
+    P.sample_rgb[3 * oid + 0] = R.col[0];
+    P.sample_rgb[3 * oid + 1] = R.col[1];
+    P.sample_rgb[3 * oid + 2] = R.col[2];
+    P.sample_cls[oid] = static_cast<uint8_t>(cls);
     if (P.records) {
-        GRRTRayRecord& o = P.records[id];
+        GRRTRayRecord& o = P.records[oid];
         o.color[0] = R.col[0];
         o.color[1] = R.col[1];
         o.color[2] = R.col[2];
@@ -171,21 +173,13 @@ __device__ __forceinline__ void finish_ray(const TraceParams& P, long long id, R
     }
 }
 
-// Loop-invariant scalars of the hot loop, read once per kernel.
-struct Hot {
-    KerrK K;
-    double r_term, r_escape, tol;
-    int max_steps, refill_min, disk, chunk;
-    long long n_rays;
-};
-
 // Termination test of src/geodesic_tracer.cpp:52-63 on the current state;
 // -1 while the ray continues.
-__device__ __forceinline__ int termination(const Hot& Q, const Ray& R) {
-    if (R.steps >= Q.max_steps) return 2;
+__device__ __forceinline__ int termination(int max_steps, double r_term, double r_escape, const Ray& R) {
+    if (R.steps >= max_steps) return 2;
     const double r = R.y.v[0];
-    if (r < Q.r_term) return 0;
-    if (r > Q.r_escape) return 1;
+    if (r < r_term) return 0;
+    if (r > r_escape) return 1;
     return -1;
 }
 
@@ -202,32 +196,49 @@ __device__ __forceinline__ unsigned lanes_below(unsigned long long mask) {
                                      __builtin_amdgcn_mbcnt_lo(static_cast<unsigned>(mask), 0u));
 }
 
-__global__ void reset_counters(DevCounters* c) {
-    if (threadIdx.x < sizeof(DevCounters) / 8)
-        reinterpret_cast<unsigned long long*>(c)[threadIdx.x] = 0ull;
+// View owning global queue index m (views sorted by q_offset).
+__device__ __forceinline__ int find_view(const ViewParams* views, int n, unsigned long long m) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (static_cast<unsigned long long>(views[mid].q_offset) <= m) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
+
 // The parameter block lives in device memory; the hot scalars are read once
-// at entry, everything else where it is used.
+// at entry, everything else where it is used.  kBatch: the launch covers
+// several views (each with its own spin, camera and disk), concatenated in
+// queue order; every lane carries its own view's metric constants.
+template <bool kBatch>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void trace_thin(const TraceParams* Pg) {
-    __shared__ double s_flux[kFluxN];
+    __shared__ double s_flux[kBatch ? 1 : kFluxN];
     __shared__ double s_spec[3 * kSpecN];
-    const Hot Q{{Pg->M, Pg->a, Pg->a * Pg->a, 2.0 * Pg->M}, Pg->r_term, Pg->r_escape, Pg->tol,
-                Pg->max_steps, Pg->refill_min, Pg->disk, Pg->chunk, Pg->n_rays};
+    const double r_escape = Pg->r_escape, tol = Pg->tol;
+    const int max_steps = Pg->max_steps, refill_min = Pg->refill_min, chunk = Pg->chunk;
+    const long long n_queue = Pg->n_rays;
     DevCounters* const counters = Pg->counters;
-    if (Q.disk) {
-        const double* fl = Pg->flux_lut;
-        const double* sl = Pg->spec_lut;
+    const int use_disk = kBatch ? 1 : Pg->view.disk;
+    if (!kBatch && use_disk) {
+        const double* fl = Pg->view.flux_lut;
         for (int i = threadIdx.x; i < kFluxN; i += blockDim.x) s_flux[i] = fl[i];
+    }
+    {
+        const double* sl = Pg->spec_lut;
         for (int i = threadIdx.x; i < 3 * kSpecN; i += blockDim.x) s_spec[i] = sl[i];
     }
     __syncthreads();
 
-    const KerrK& K = Q.K;
+    // Metric constants: uniform for a single view, per lane in a batch.
+    KerrK K{Pg->view.M, Pg->view.a, Pg->view.a * Pg->view.a, 2.0 * Pg->view.M};
+    double r_term = Pg->view.r_term;
+    int vi = 0;
     const unsigned lane = lane_id();
     Ray R;
-    long long id = -1;
+    long long oid = -1;
     bool live = false;
     unsigned long long n_steps = 0, n_att = 0, n_rays = 0;
 
@@ -237,8 +248,8 @@ void trace_thin(const TraceParams* Pg) {
     // integration; the chunk is read only when the current range runs dry.
     // A cancel request stops further claims; claimed rays always complete,
     // so the finished rays are exactly [0, queue_head).
-    const unsigned long long nR = static_cast<unsigned long long>(Q.n_rays);
-    const unsigned long long CH = static_cast<unsigned long long>(Q.chunk);
+    const unsigned long long nR = static_cast<unsigned long long>(n_queue);
+    const unsigned long long CH = static_cast<unsigned long long>(chunk);
     const volatile int* cancel = Pg->cancel;
     unsigned long long cur = 0, cur_end = 0;     // wave-uniform claimed range
     unsigned long long pre = 0;                  // lane 0: base of the claim in flight
@@ -254,7 +265,7 @@ void trace_thin(const TraceParams* Pg) {
         // ---- refill idle lanes (wave-uniform branch) ----
         const unsigned long long idle = __ballot(!live);
         const int n_idle = __popcll(idle);
-        if (queue_open && (n_idle >= Q.refill_min || n_idle == 64)) {
+        if (queue_open && (n_idle >= refill_min || n_idle == 64)) {
             // cur never passes cur_end (see the advance below), so avail >= 0.
             const unsigned long long avail = cur_end > cur ? cur_end - cur : 0ull;
             unsigned long long nb = ~0ull;
@@ -279,15 +290,24 @@ void trace_thin(const TraceParams* Pg) {
                 if (k < avail) mine = cur + k;
                 else if (nb != ~0ull) mine = nb + (k - avail);
                 if (mine < nR) {
-                    id = static_cast<long long>(mine);
                     keep_loads_here();
-                    init_ray(*Pg, K, id, R);
-                    live = true;
-                    const int term = termination(Q, R);
-                    if (term >= 0) {
-                        finish_ray(*Pg, id, R, term);
-                        live = false;
-                        ++n_rays;
+                    if (kBatch) vi = find_view(Pg->views, Pg->n_views, mine);
+                    const ViewParams& V = kBatch ? Pg->views[vi] : Pg->view;
+                    SampleId d;
+                    if (decode_ray(V, static_cast<long long>(mine) - V.q_offset, d)) {
+                        if (kBatch) {
+                            K = KerrK{V.M, V.a, V.a * V.a, 2.0 * V.M};
+                            r_term = V.r_term;
+                        }
+                        oid = V.ray_offset + d.local;
+                        init_ray(V, d, K, R);
+                        live = true;
+                        const int term = termination(max_steps, r_term, r_escape, R);
+                        if (term >= 0) {
+                            finish_ray(*Pg, oid, R, term);
+                            live = false;
+                            ++n_rays;
+                        }
                     }
                 }
             }
@@ -369,13 +389,13 @@ void trace_thin(const TraceParams* Pg) {
         if (R.retry >= 20) {
             accept = true;           // retries exhausted: take the last attempt
             next = h;
-        } else if (err <= Q.tol) {
+        } else if (err <= tol) {
             accept = true;
-            double f = (err > 1e-30) ? step_factor(Q.tol / err) : 5.0;
+            double f = (err > 1e-30) ? step_factor(tol / err) : 5.0;
             f = sclamp(f, 1.0, 5.0);
             next = smin(h * f, 5.0 * smax(__builtin_fabs(y5.v[0]), 1.0));
         } else {
-            double f = step_factor(Q.tol / err);
+            double f = step_factor(tol / err);
             f = sclamp(f, 0.2, 1.0);
             const double dl = h * f;
             if (dl < 1e-6) {
@@ -394,15 +414,16 @@ void trace_thin(const TraceParams* Pg) {
         R.retry = 0;
 
         // ---- thin-disk crossing between the previous and new state ----
-        if (Q.disk) {
+        if (use_disk) {
             const double half_pi = kPi / 2.0;
             const double dp = R.y.v[1] - half_pi, dn = y5.v[1] - half_pi;
             if (dp * dn < 0.0 && __builtin_fabs(dp - dn) > 1e-12) {
                 const double frac = -dp / (dn - dp);
                 const double rc = R.y.v[0] + frac * (y5.v[0] - R.y.v[0]);
                 keep_loads_here();
-                if (rc >= Pg->r_in && rc <= Pg->r_out)
-                    disk_emission(*Pg, s_flux, s_spec, rc, R.P0, R.P3, R);
+                const ViewParams& V = kBatch ? Pg->views[vi] : Pg->view;
+                if (V.disk && rc >= V.r_in && rc <= V.r_out)
+                    disk_emission(V, kBatch ? V.flux_lut : s_flux, s_spec, rc, R.P0, R.P3, R);
             }
         }
         R.y = y5;
@@ -410,13 +431,13 @@ void trace_thin(const TraceParams* Pg) {
         ++R.steps;
         R.h_last = __builtin_fabs(hamiltonian_from_rhs(R, dt7));
         R.h_max = smax(R.h_max, R.h_last);
-        const int term = termination(Q, R);
+        const int term = termination(max_steps, r_term, r_escape, R);
         if (term >= 0) {
             n_steps += R.steps;
             n_att += R.attempts;
             ++n_rays;
             keep_loads_here();
-            finish_ray(*Pg, id, R, term);
+            finish_ray(*Pg, oid, R, term);
             live = false;
         }
     }
@@ -435,15 +456,16 @@ void trace_thin(const TraceParams* Pg) {
 }
 
 // Average the samples of each pixel in sample order (src/renderer.cpp:56-72)
-// and write RGBA float32 for the window.  Pixels with a sample beyond the
-// last claimed ray (a cancelled render) are written as zero.
-__global__ void finalize_pixels(const double* __restrict__ rgb, float* __restrict__ out,
-                                long long npix, int nS, double inv, const DevCounters* cnt) {
+// and write RGBA float32.  A pixel with a sample that was never traced (a
+// cancelled render leaves the 0xFF class sentinel) is written as zero.
+__global__ void finalize_pixels(const double* __restrict__ rgb, const uint8_t* __restrict__ cls,
+                                float* __restrict__ out, long long npix, int nS, double inv) {
     const long long p = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (p >= npix) return;
-    const unsigned long long done = cnt->queue_head;
+    bool done = true;
+    for (int s = 0; s < nS; ++s) done = done && cls[p * nS + s] != 0xFF;
     float4 o;
-    if (static_cast<unsigned long long>((p + 1) * nS) > done) {
+    if (!done) {
         o.x = o.y = o.z = 0.0f;
     } else {
         double a0 = 0.0, a1 = 0.0, a2 = 0.0;
@@ -461,27 +483,37 @@ __global__ void finalize_pixels(const double* __restrict__ rgb, float* __restric
     reinterpret_cast<float4*>(out)[p] = o;
 }
 
+// Zero the counters and mark every sample untraced.
+__global__ void reset_launch(DevCounters* c, uint8_t* __restrict__ cls, long long n) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < static_cast<long long>(sizeof(DevCounters) / 8)) reinterpret_cast<unsigned long long*>(c)[i] = 0ull;
+    const long long stride = static_cast<long long>(gridDim.x) * blockDim.x;
+    for (long long k = i; k < n; k += stride) cls[k] = 0xFF;
+}
+
 }  // namespace grrt_amd
 
 // Host-callable launchers (C++ linkage inside the library).
 namespace grrt_amd {
 
-hipError_t launch_reset(DevCounters* c, hipStream_t st) {
-    hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, st, c);
+hipError_t launch_reset(DevCounters* c, uint8_t* cls, long long n_rays, hipStream_t st) {
+    const long long blocks = std::min<long long>(std::max<long long>((n_rays + 255) / 256, 1), 4096);
+    hipLaunchKernelGGL(reset_launch, dim3(static_cast<unsigned>(blocks)), dim3(256), 0, st, c, cls, n_rays);
     return hipGetLastError();
 }
 
-hipError_t launch_trace_thin(const TraceParams* P, int blocks, int threads, hipStream_t st) {
-    hipLaunchKernelGGL(trace_thin, dim3(blocks), dim3(threads), 0, st, P);
+hipError_t launch_trace_thin(const TraceParams* P, bool batch, int blocks, int threads, hipStream_t st) {
+    if (batch) hipLaunchKernelGGL(trace_thin<true>, dim3(blocks), dim3(threads), 0, st, P);
+    else hipLaunchKernelGGL(trace_thin<false>, dim3(blocks), dim3(threads), 0, st, P);
     return hipGetLastError();
 }
 
-hipError_t launch_finalize(const double* rgb, float* out, long long npix, int nS,
-                           const DevCounters* cnt, hipStream_t st) {
+hipError_t launch_finalize(const double* rgb, const uint8_t* cls, float* out, long long npix, int nS,
+                           hipStream_t st) {
     const int T = 256;
     const long long B = (npix + T - 1) / T;
-    hipLaunchKernelGGL(finalize_pixels, dim3(static_cast<unsigned>(B)), dim3(T), 0, st, rgb, out,
-                       npix, nS, 1.0 / nS, cnt);
+    hipLaunchKernelGGL(finalize_pixels, dim3(static_cast<unsigned>(B)), dim3(T), 0, st, rgb, cls, out,
+                       npix, nS, 1.0 / nS);
     return hipGetLastError();
 }
 

@@ -187,7 +187,7 @@ __device__ MarchOut raymarch(const TraceParams& P, const int* perm, const KerrK&
     const VolDev& V = P.vol;
     const int nb = kSpectral ? P.nbins : 3;
     const int med = kSpectral ? P.nbins / 2 : 1;
-    const double ut_obs = 1.0 / sqrt(1.0 - 2.0 / P.r_obs);
+    const double ut_obs = 1.0 / sqrt(1.0 - 2.0 / P.view.r_obs);
     MarchOut mo{0, 0};
     double r = st.y.v[0];
     const double z0 = r * cos(st.y.v[1]);
@@ -273,22 +273,23 @@ __global__ __launch_bounds__(256) void trace_vol(const TraceParams* Pg) {
     for (int i = threadIdx.x; i < 512; i += blockDim.x) s_perm[i] = Pg->vol.perm[i];
     __syncthreads();
     const TraceParams& P = *Pg;
-    // Not a persistent kernel: every ray is claimed by its launch.
-    if (blockIdx.x == 0 && threadIdx.x == 0) P.counters->queue_head = static_cast<unsigned long long>(P.n_rays);
     const long long id = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (id >= P.n_rays) return;
     const VolDev& V = P.vol;
-    const KerrK K{P.M, P.a, P.a * P.a, 2.0 * P.M};
+    const ViewParams& Vw = P.view;
+    const KerrK K{Vw.M, Vw.a, Vw.a * Vw.a, 2.0 * Vw.M};
     GState s;
-    camera_ray(P, id, s.P0, s.P3, s.y);
+    SampleId sid;
+    decode_ray(Vw, id, sid);      // row-major order: every id is a sample of the window
+    camera_ray(Vw, sid, s.P0, s.P3, s.y);
     Bins<RT, kSpectral> b(P, id);
     const int nb = kSpectral ? P.nbins : 3;
     const double half_pi = kPi / 2.0;
-    double dl = 0.01 * P.r_obs;
+    double dl = 0.01 * Vw.r_obs;
     int term = 2, steps = 0, attempts = 0, entries = 0, emitted = 0, march_steps = 0;
     for (int step = 0; step < P.max_steps; ++step) {
         const double r = s.y.v[0];
-        if (r < P.r_term) { term = 0; break; }
+        if (r < Vw.r_term) { term = 0; break; }
         if (r > P.r_escape) { term = 1; break; }
         const GState prev = s;
         {

@@ -60,6 +60,8 @@ typedef struct {
 GRRT_EXPORT int grrt_ext_set_sampling(GRRTContext* ctx, int mode, int spp, uint64_t seed);
 /* Keep per-ray records of subsequent renders (enable=1) for grrt_ext_get_records. */
 GRRT_EXPORT int grrt_ext_set_records(GRRTContext* ctx, int enable);
+/* Samples of the context's frame (width * height * samples per pixel), or
+ * of all views when the last render was a batch (grrt_ext_render_batch). */
 GRRT_EXPORT int64_t grrt_ext_num_samples(const GRRTContext* ctx);
 GRRT_EXPORT int grrt_ext_get_records(GRRTContext* ctx, GRRTRayRecord* out, int64_t n);
 GRRT_EXPORT int grrt_ext_get_classes(GRRTContext* ctx, uint8_t* out, int64_t n);
@@ -70,9 +72,42 @@ GRRT_EXPORT int grrt_ext_render_device(GRRTContext* ctx, float* d_rgba);
 /* Persistent-kernel launch shape: threads per block (multiple of 64),
  * blocks per CU, the number of idle lanes (1..64) that triggers a
  * wavefront refill, and the rays a wavefront claims from the device queue
- * per atomic (>= 64).  0 keeps the current value. */
+ * per atomic (>= 64); 0 keeps the current value.  tile: queue order of the
+ * RGB frame, 0 = row-major samples, 2..32 = tile x tile pixel blocks;
+ * negative keeps the current order. */
 GRRT_EXPORT int grrt_ext_set_tuning(GRRTContext* ctx, int block, int blocks_per_cu, int refill_min,
-                                    int chunk);
+                                    int chunk, int tile);
+
+/* One camera view of a batch: its own Kerr spin (dimensionless, a/M, used
+ * as given: 0 is Schwarzschild, negative is a retrograde hole, unlike
+ * GRRTParams.spin where <= 0 selects 0.998), observer position, field of
+ * view and resolution (non-positive r, theta, fov take the context's
+ * values).  Everything else (mass,
+ * thin disk settings with r_in = ISCO of this spin unless disk_inner is set,
+ * stars, tolerance, step limit, sample placement) comes from the context. */
+typedef struct {
+    double spin;
+    double observer_r, observer_theta, observer_phi, fov;
+    int width, height;
+} GRRTView;
+
+/* Renders n views of uneven size in ONE persistent kernel launch over the
+ * concatenated ray queue (one lane per ray, per-lane view constants).  The
+ * launch sequence (parameter upload, reset, trace, finalize) is recorded as
+ * a HIP graph and replayed while the views stay the same.  out: the RGBA
+ * float32 frames in view order, sum(width*height*4) floats; host memory,
+ * or device memory when device_out != 0. */
+GRRT_EXPORT int grrt_ext_render_batch(GRRTContext* ctx, const GRRTView* views, int n, float* out,
+                                      int device_out);
+
+/* Picks the persistent kernel's launch shape by timing candidates on the
+ * real workload: the context's frame (views == NULL) or the given batch.
+ * Candidates: pixel tile (0 = row order, 8), blocks per CU (1, 2), refill
+ * threshold (2, 4, 8, 16 idle lanes), queue chunk (64, 128).  The best is
+ * kept (for the batch: recorded with its graph) and written to
+ * best[4] = {tile, blocks_per_cu, refill_min, chunk}; best_ms its time. */
+GRRT_EXPORT int grrt_ext_autotune(GRRTContext* ctx, const GRRTView* views, int n, int best[4],
+                                  double* best_ms);
 
 /* Volumetric path precision: 0 = all FP64 (default), 1 = FP32 radiative
  * transfer (opacities, source, extinction) with compensated summation of

@@ -1,0 +1,155 @@
+"""Batched rendering (grrt_ext_render_batch): several views of uneven size and
+different spin, observer and field of view traced in one persistent launch
+over the concatenated ray queue, recorded once as a HIP graph and replayed.
+
+Every frame of a batch must equal the same view rendered alone (same
+integrator, same per-ray arithmetic: only the queue order differs) and agree
+with the CPU oracle to the thin-path bars; replays of the recorded graph are
+bit-identical; the autotuner keeps a launch shape that renders the same
+frames.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from parity_util import class_boundary_mask, median_rel_err
+
+pytestmark = pytest.mark.gpu
+
+# (spin, r_obs, theta, phi, fov, width, height): uneven sizes, a prograde
+# and a retrograde hole, a Schwarzschild view, different inclinations.
+VIEWS = [
+    (0.998, 50.0, math.radians(80.0), 0.0, math.radians(90.0), 96, 64),
+    (0.0, 30.0, math.radians(60.0), 0.0, math.radians(70.0), 40, 72),
+    (-0.5, 80.0, math.radians(88.0), 0.3, math.radians(40.0), 128, 48),
+    (0.7, 20.0, math.radians(30.0), 0.0, math.radians(110.0), 17, 23),
+]
+
+
+def _base(grrt, **kw):
+    return grrt.make_params(width=64, height=64, metric=1, spin=0.5, observer_r=50.0,
+                            observer_theta=math.radians(80.0), fov=math.radians(90.0), disk=True,
+                            disk_outer=20.0, stars=kw.pop("stars", False), **kw)
+
+
+def _single(grrt, v, stars=False, spp=1, mode=None):
+    """The view rendered alone through the reference ABI.  GRRTParams maps a
+    Kerr spin <= 0 to 0.998 (src/api.cpp:57), so a = 0 is the Schwarzschild
+    metric there and a retrograde hole has no single-view form."""
+    spin, r, th, ph, fov, w, h = v
+    assert spin >= 0.0
+    p = grrt.make_params(width=w, height=h, metric=1 if spin > 0 else 0, spin=spin, observer_r=r, observer_theta=th,
+                         observer_phi=ph, fov=fov, disk=True, disk_outer=20.0, stars=stars)
+    with grrt.Context(p) as ctx:
+        ctx.set_sampling(grrt.SAMPLING_CENTER if mode is None else mode, spp, 7)
+        fb = ctx.render()
+        cls = ctx.classes()
+    return fb, cls
+
+
+def test_batch_equals_single_views(grrt):
+    print("case: batch of 4 uneven views vs single-view renders, pixel centres", flush=True)
+    with grrt.Context(_base(grrt)) as ctx:
+        ctx.set_sampling(grrt.SAMPLING_CENTER)
+        frames = ctx.render_batch(VIEWS)
+        cls = ctx.classes()
+        st = ctx.stats()
+    assert st.rays == sum(v[5] * v[6] for v in VIEWS)
+    o = 0
+    res = []
+    for v, fb in zip(VIEWS, frames):
+        n = v[5] * v[6]
+        c = cls[o:o + n]
+        o += n
+        if v[0] < 0:
+            continue            # retrograde: covered by the oracle parity test
+        ref, rcls = _single(grrt, v)
+        assert fb.shape == ref.shape
+        agree = np.mean(c == rcls)
+        diff = np.abs(fb - ref).max(-1) > 1e-9 * np.maximum(np.abs(ref).max(-1), 1e-300)
+        print(f"view {v[:1]} {v[5]}x{v[6]}: class agreement {agree:.5f}, "
+              f"pixels differing {diff.mean():.5f}", flush=True)
+        res.append((agree, diff.mean()))
+    for agree, frac in res:
+        assert agree >= 0.999
+        assert frac <= 0.002
+
+
+def test_batch_oracle_parity(grrt, oracle):
+    """Each view of the batch against the oracle scene of that view."""
+    print("case: batch of 4 views vs oracle, pixel centres", flush=True)
+    with grrt.Context(_base(grrt)) as ctx:
+        ctx.set_sampling(grrt.SAMPLING_CENTER)
+        frames = ctx.render_batch(VIEWS)
+        cls = ctx.classes()
+    o = 0
+    for v, fb in zip(VIEWS, frames):
+        spin, r, th, ph, fov, w, h = v
+        sc = oracle.scene(width=w, height=h, mass=1.0, spin_a=spin, r_obs=r, theta_obs=th,
+                          phi_obs=ph, fov=fov, disk=True, r_out=20.0, stars=False)
+        ofb, ocls, _ = oracle.render(sc, oracle.JITTER_CENTER, 1, threads=0)
+        c = cls[o:o + w * h]
+        o += w * h
+        agree = np.mean(c == ocls)
+        mism = (c != ocls).reshape(h, w)
+        edge = class_boundary_mask(ocls.reshape(h, w))
+        same = (c == ocls).reshape(h, w)
+        mre = median_rel_err(fb[..., :3][same], ofb[..., :3][same])
+        print(f"view spin={spin} {w}x{h}: agreement {agree:.5f} median rel {mre:.2e}", flush=True)
+        assert agree >= 0.999 or not np.any(mism & ~edge)
+        assert not np.any(mism & ~edge)
+        assert mre <= 1e-6
+
+
+def test_batch_graph_replay_and_rebuild(grrt):
+    """Replays of the recorded graph are bit-identical; a changed batch is
+    re-recorded; the sampled modes place samples per view."""
+    print("case: batch graph replay x3, then a smaller batch, stars + 4 spp hash", flush=True)
+    with grrt.Context(_base(grrt, stars=True, spp=4)) as ctx:
+        first = ctx.render_batch(VIEWS)
+        for _ in range(3):
+            again = ctx.render_batch(VIEWS)
+            for a, b in zip(first, again):
+                assert np.array_equal(a, b)
+        small = ctx.render_batch(VIEWS[1:3])
+        for a, b in zip(first[1:3], small):
+            assert np.array_equal(a, b)
+        st = ctx.stats()
+        assert st.rays == 4 * sum(v[5] * v[6] for v in VIEWS[1:3])
+    for v, fb in zip(VIEWS[:2], first[:2]):
+        ref, _ = _single(grrt, v, stars=True, spp=4, mode=grrt.SAMPLING_HASH)
+        diff = np.abs(fb - ref).max(-1) > 1e-9 * np.maximum(np.abs(ref).max(-1), 1e-300)
+        assert diff.mean() <= 0.002, diff.mean()
+
+
+def test_autotune(grrt):
+    print("case: autotune on a 3-view batch and on a 128^2 frame", flush=True)
+    with grrt.Context(_base(grrt)) as ctx:
+        ctx.set_sampling(grrt.SAMPLING_CENTER)
+        ref = ctx.render_batch(VIEWS[:3])
+        best, ms = ctx.autotune(VIEWS[:3])
+        print("batch best", best, f"{ms:.3f} ms", flush=True)
+        assert best["tile"] in (0, 8) and best["blocks_per_cu"] in (1, 2)
+        assert best["refill_min"] in (2, 4, 8, 16) and best["chunk"] in (64, 128)
+        assert ms > 0
+        got = ctx.render_batch(VIEWS[:3])
+        for a, b in zip(ref, got):
+            assert np.array_equal(a, b)
+    p = grrt.make_params(width=128, height=128, metric=1, spin=0.9, disk=True, disk_outer=20.0)
+    with grrt.Context(p) as ctx:
+        ctx.set_sampling(grrt.SAMPLING_CENTER)
+        ref = ctx.render()
+        best, ms = ctx.autotune()
+        print("frame best", best, f"{ms:.3f} ms", flush=True)
+        assert np.array_equal(ref, ctx.render())
+
+
+def test_batch_rejects_bad_views(grrt):
+    with grrt.Context(_base(grrt)) as ctx:
+        with pytest.raises(RuntimeError):
+            ctx.render_batch([(1.2, 50.0, 1.3, 0.0, 1.5, 8, 8)])
+        with pytest.raises(RuntimeError):
+            ctx.render_batch([(0.5, 50.0, 1.3, 0.0, 1.5, 0, 8)])
+        ok = ctx.render_batch([(0.5, 50.0, 1.3, 0.0, 1.5, 8, 8)])
+        assert ok[0].shape == (8, 8, 4)
