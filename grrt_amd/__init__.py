@@ -98,6 +98,37 @@ def views_array(views) -> "C.Array":
     return arr
 
 
+class GRRTTrainConfig(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int), ("batch", C.c_int), ("lr_max", C.c_double), ("lr_min", C.c_double),
+        ("warmup", C.c_int), ("pad", C.c_int), ("w_r", C.c_double), ("w_g", C.c_double),
+        ("w_dir", C.c_double), ("seed", C.c_uint64),
+    ]
+
+
+class GRRTTrainStats(C.Structure):
+    _fields_ = [
+        ("steps", C.c_int64), ("rays", C.c_int64), ("loss", C.c_double), ("ce", C.c_double),
+        ("accuracy", C.c_double), ("rmse_lnr", C.c_double), ("rmse_lng", C.c_double),
+        ("rmse_dir", C.c_double), ("ms", C.c_double),
+    ]
+
+
+class GRRTSurrogateStats(C.Structure):
+    _fields_ = [
+        ("infer_ms", C.c_double), ("trace_ms", C.c_double), ("judge_ms", C.c_double),
+        ("total_ms", C.c_double), ("rays", C.c_int64), ("fallback_rays", C.c_int64),
+        ("judged_rays", C.c_int64), ("trace_attempts", C.c_int64),
+        ("judge_class_agreement", C.c_double), ("judge_median_rel_err", C.c_double),
+    ]
+
+
+def _as_dict(st: C.Structure) -> dict:
+    return {f: getattr(st, f) for f, _ in st._fields_ if f != "pad"}
+
+
+SURROGATE_PARAMS = 201225
+
 PROGRESS_FN = C.CFUNCTYPE(None, C.c_float, C.c_void_p)
 
 # Every symbol the C headers declare: (name, restype, argtypes).
@@ -134,6 +165,16 @@ API = [
     ("grrt_ext_autotune", C.c_int, [_P, C.POINTER(GRRTView), C.c_int, C.POINTER(C.c_int),
                                     C.POINTER(C.c_double)]),
     ("grrt_ext_set_precision", C.c_int, [_P, C.c_int]),
+    ("grrt_ext_surrogate_init", C.c_int, [_P, C.c_uint64]),
+    ("grrt_ext_surrogate_generate", C.c_int, [_P, C.c_int64, C.c_uint64, C.c_int]),
+    ("grrt_ext_surrogate_train", C.c_int, [_P, C.POINTER(GRRTTrainConfig), C.POINTER(GRRTTrainStats)]),
+    ("grrt_ext_surrogate_evaluate", C.c_int, [_P, C.POINTER(GRRTTrainStats)]),
+    ("grrt_ext_surrogate_params", C.c_int, [_P, _P, C.c_int64, C.c_int]),
+    ("grrt_ext_surrogate_forward", C.c_int, [_P, _P, C.c_int64, _P]),
+    ("grrt_ext_surrogate_gradients", C.c_int, [_P, _P, _P, C.c_int64, _P, _P, C.POINTER(C.c_double)]),
+    ("grrt_ext_surrogate_dataset", C.c_int, [_P, C.c_int, _P, _P, C.c_int64]),
+    ("grrt_ext_render_surrogate", C.c_int, [_P, _P, C.c_int, C.c_double, C.c_double, C.c_double,
+                                            C.POINTER(GRRTSurrogateStats)]),
     ("grrt_ext_render_spectral_f32", C.c_int, [_P, C.POINTER(C.c_float), C.c_int, C.c_int]),
     ("grrt_ext_vol_build", _P, [C.POINTER(GRRTParams)]),
     ("grrt_ext_vol_scalars", C.c_int, [_P, C.POINTER(C.c_double), C.c_int]),
@@ -302,6 +343,73 @@ class Context:
             frames.append(out[o:o + n].reshape(v.height, v.width, 4))
             o += n
         return frames
+
+    # ---- learned surrogate ----
+    def surrogate_init(self, seed: int = 0):
+        self._check(self._L.grrt_ext_surrogate_init(self._h, seed), "surrogate_init")
+
+    def surrogate_generate(self, n_rays: int, seed: int = 0, validation: bool = False):
+        self._check(self._L.grrt_ext_surrogate_generate(self._h, n_rays, seed, 1 if validation else 0),
+                    "surrogate_generate")
+
+    def surrogate_train(self, steps, batch=4096, lr_max=1e-3, lr_min=1e-5, warmup=100, w_r=10.0, w_g=10.0,
+                        w_dir=1.0, seed=0) -> dict:
+        cfg = GRRTTrainConfig(steps=steps, batch=batch, lr_max=lr_max, lr_min=lr_min, warmup=warmup,
+                              w_r=w_r, w_g=w_g, w_dir=w_dir, seed=seed)
+        st = GRRTTrainStats()
+        self._check(self._L.grrt_ext_surrogate_train(self._h, C.byref(cfg), C.byref(st)), "surrogate_train")
+        return _as_dict(st)
+
+    def surrogate_evaluate(self) -> dict:
+        st = GRRTTrainStats()
+        self._check(self._L.grrt_ext_surrogate_evaluate(self._h, C.byref(st)), "surrogate_evaluate")
+        return _as_dict(st)
+
+    def surrogate_params(self, values: np.ndarray | None = None) -> np.ndarray:
+        if values is None:
+            out = np.zeros(SURROGATE_PARAMS, np.float32)
+            self._check(self._L.grrt_ext_surrogate_params(self._h, out.ctypes.data, out.size, 0), "params")
+            return out
+        v = np.ascontiguousarray(values, np.float32)
+        self._check(self._L.grrt_ext_surrogate_params(self._h, v.ctypes.data, v.size, 1), "params")
+        return v
+
+    def surrogate_forward(self, x: np.ndarray) -> np.ndarray:
+        xx = np.ascontiguousarray(x, np.float32)
+        y = np.zeros((xx.shape[0], 9), np.float32)
+        self._check(self._L.grrt_ext_surrogate_forward(self._h, xx.ctypes.data, xx.shape[0], y.ctypes.data),
+                    "surrogate_forward")
+        return y
+
+    def surrogate_gradients(self, x, t, w=(1.0, 1.0, 1.0)):
+        xx, tt = np.ascontiguousarray(x, np.float32), np.ascontiguousarray(t, np.float32)
+        ww = np.ascontiguousarray(w, np.float32)
+        g = np.zeros(SURROGATE_PARAMS, np.float32)
+        loss = C.c_double()
+        self._check(self._L.grrt_ext_surrogate_gradients(self._h, xx.ctypes.data, tt.ctypes.data, xx.shape[0],
+                                                          ww.ctypes.data, g.ctypes.data, C.byref(loss)),
+                    "surrogate_gradients")
+        return loss.value, g
+
+    def surrogate_dataset(self, n: int, validation: bool = False):
+        x = np.zeros((n, 5), np.float32)
+        t = np.zeros((n, 8), np.float32)
+        self._check(self._L.grrt_ext_surrogate_dataset(self._h, 1 if validation else 0, x.ctypes.data,
+                                                        t.ctypes.data, n), "surrogate_dataset")
+        return x, t
+
+    def render_surrogate(self, entropy_max=0.1, judge_fraction=0.02, lng_tol=0.0, d_ptr: int | None = None):
+        """Surrogate frame with FP64 fallback and judge; returns (frame or
+        None when rendering into device memory, stats dict)."""
+        st = GRRTSurrogateStats()
+        if d_ptr is not None:
+            self._check(self._L.grrt_ext_render_surrogate(self._h, d_ptr, 1, entropy_max, lng_tol,
+                                                           judge_fraction, C.byref(st)), "render_surrogate")
+            return None, _as_dict(st)
+        fb = np.zeros((self.params.height, self.params.width, 4), np.float32)
+        self._check(self._L.grrt_ext_render_surrogate(self._h, fb.ctypes.data, 0, entropy_max, lng_tol,
+                                                       judge_fraction, C.byref(st)), "render_surrogate")
+        return fb, _as_dict(st)
 
     def autotune(self, views=None) -> tuple[dict, float]:
         """Time launch shapes on the frame (views None) or the batch and keep

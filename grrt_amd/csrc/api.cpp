@@ -13,6 +13,7 @@
 #include <string>
 #include <thread>
 
+#include "api_internal.hpp"
 #include "context.hpp"
 #include "grrt/api.h"
 #include "grrt_ext.h"
@@ -28,27 +29,10 @@ static_assert(offsetof(GRRTParams, mass) == 16 && offsetof(GRRTParams, cam_yaw) 
               "GRRTParams field offsets must match the reference");
 static_assert(sizeof(GRRTRayRecord) == 144, "GRRTRayRecord layout");
 
-namespace {
+namespace grrt_api {
 
 thread_local std::string g_last_error;
 
-struct HipFailure : std::exception {
-    std::string msg;
-    explicit HipFailure(std::string m) : msg(std::move(m)) {}
-    const char* what() const noexcept override { return msg.c_str(); }
-};
-
-void check(hipError_t e, const char* what) {
-    if (e != hipSuccess) throw HipFailure(std::string(what) + ": " + hipGetErrorString(e));
-}
-
-// Set-up time upload of a host table (synchronous: the source may be a temporary).
-template <class T>
-void upload(grrt_amd::DevBuf<T>& b, const std::vector<T>& v, hipStream_t st) {
-    (void)st;
-    check(b.ensure(v.size()), "hipMalloc(table)");
-    check(hipMemcpy(b.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy(table)");
-}
 
 // Volumetric disk parameters exactly as the reference wires them
 // (src/api.cpp:79-127): the disk takes the raw mass and spin fields, its
@@ -285,7 +269,7 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
     check(hipEventRecord(ctx->ev[0], ctx->stream), "hipEventRecord");
     if (vol) check(launch_trace_vol(ctx->params_dev.ptr, n_rays, spectral, ctx->mixed_precision, ctx->stream),
                    "volumetric trace launch");
-    else check(launch_trace_thin(ctx->params_dev.ptr, false, blocks, threads, ctx->stream), "trace kernel launch");
+    else check(launch_trace_thin(ctx->params_dev.ptr, 0, blocks, threads, ctx->stream), "trace kernel launch");
     check(hipEventRecord(ctx->ev[1], ctx->stream), "hipEventRecord");
     if (spectral)
         check(launch_finalize_spectral(P.sample_spec, d_out, npix, nS, nb, kind == Output::kSpectralF32, ctx->stream),
@@ -386,21 +370,9 @@ void tonemap_host(float* fb, int w, int h) {
     }
 }
 
-template <class F>
-int guarded(GRRTContext* ctx, F&& f) {
-    if (!ctx) { g_last_error = "null context"; return -1; }
-    try {
-        return f();
-    } catch (const std::exception& e) {
-        ctx->error_msg = e.what();
-        return -1;
-    } catch (...) {
-        ctx->error_msg = "unknown error";
-        return -1;
-    }
-}
+}  // namespace grrt_api
 
-}  // namespace
+using namespace grrt_api;
 
 extern "C" {
 
@@ -456,6 +428,9 @@ void grrt_destroy(GRRTContext* ctx) {
     ctx->params_dev.release();
     if (ctx->batch_exec) (void)hipGraphExecDestroy(ctx->batch_exec);
     if (ctx->batch_graph) (void)hipGraphDestroy(ctx->batch_graph);
+    grrt_amd::sg::destroy_state(ctx->sg);
+    ctx->sg = nullptr;
+    ctx->batch_records.release();
     ctx->batch_params.release();
     ctx->batch_tp.release();
     ctx->batch_rgb.release();
@@ -625,6 +600,7 @@ int grrt_ext_set_sampling(GRRTContext* ctx, int mode, int spp, uint64_t seed) {
 int grrt_ext_set_records(GRRTContext* ctx, int enable) {
     return guarded(ctx, [&] {
         ctx->keep_records = enable != 0;
+        ctx->batch_views.clear();      // the records buffer is recorded with the batch
         ctx->records_valid = false;
         return 0;
     });
@@ -794,11 +770,11 @@ int grrt_ext_opacity(double nu, double rho, double T, double out[6]) {
 
 }  // extern "C"
 
-namespace {
+namespace grrt_api {
 
 // (Re)build the batch's per-view tables and parameter blocks, and record
 // the launch sequence as a graph.
-void prepare_batch(GRRTContext* ctx, const GRRTView* views, int n) {
+void prepare_batch(GRRTContext* ctx, const GRRTView* views, int n, uint64_t seed_stride) {
     using namespace grrt_amd;
     const Resolved& r = ctx->res;
     if (r.volumetric) throw HipFailure("batched rendering covers the thin disk only");
@@ -825,6 +801,7 @@ void prepare_batch(GRRTContext* ctx, const GRRTView* views, int n) {
         fill_view(ctx, vp[v], a, r_obs, th, w.observer_phi, fov, w.width, w.height, cam,
                   ctx->batch_flux.ptr + static_cast<size_t>(v) * kFluxN, fmax, r_in, 0, 0, w.width, w.height,
                   ctx->tune_tile);
+        vp[v].seed = ctx->sampling_seed + static_cast<uint64_t>(v) * seed_stride;
         vp[v].q_offset = q;
         vp[v].ray_offset = rays;
         vp[v].pix_offset = pix;
@@ -854,7 +831,8 @@ void prepare_batch(GRRTContext* ctx, const GRRTView* views, int n) {
     P.star_tpb = ctx->star_tpb.ptr;
     P.sample_rgb = ctx->batch_rgb.ptr;
     P.sample_cls = ctx->batch_cls.ptr;
-    P.records = nullptr;
+    if (ctx->keep_records) check(ctx->batch_records.ensure(static_cast<size_t>(rays)), "hipMalloc(batch records)");
+    P.records = ctx->keep_records ? ctx->batch_records.ptr : nullptr;
     P.counters = ctx->batch_counters.ptr;
     P.cancel = nullptr;
     P.refill_min = std::clamp(ctx->tune_refill, 1, 64);
@@ -869,7 +847,7 @@ void prepare_batch(GRRTContext* ctx, const GRRTView* views, int n) {
     check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
     hipError_t e = launch_reset(P.counters, P.sample_cls, rays, ctx->stream);
     if (e == hipSuccess)
-        e = launch_trace_thin(ctx->batch_tp.ptr, true, std::max(1, ctx->num_cus * ctx->tune_blocks_per_cu),
+        e = launch_trace_thin(ctx->batch_tp.ptr, 1, std::max(1, ctx->num_cus * ctx->tune_blocks_per_cu),
                               ctx->tune_block, ctx->stream);
     if (e == hipSuccess) e = launch_finalize(P.sample_rgb, P.sample_cls, ctx->batch_out.ptr, pix, nS, ctx->stream);
     hipGraph_t g = nullptr;
@@ -923,7 +901,7 @@ int run_batch(GRRTContext* ctx) {
     return 0;
 }
 
-}  // namespace
+}  // namespace grrt_api
 
 extern "C" {
 

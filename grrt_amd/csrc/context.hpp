@@ -40,10 +40,16 @@ hipError_t launch_reset(DevCounters* c, uint8_t* cls, long long n_rays, hipStrea
 hipError_t launch_trace_vol(const TraceParams* P, long long n_rays, bool spectral, bool fp32_rt, hipStream_t st);
 hipError_t launch_finalize_spectral(const double* spec, void* out, long long npix, int nS, int nb, bool f32,
                                     hipStream_t st);
-hipError_t launch_trace_thin(const TraceParams* P, bool batch, int blocks, int threads, hipStream_t st);
+// mode: 0 one view, 1 batch of views, 2 listed rays of one view.
+hipError_t launch_trace_thin(const TraceParams* P, int mode, int blocks, int threads, hipStream_t st);
 hipError_t launch_finalize(const double* rgb, const uint8_t* cls, float* out, long long npix, int nS,
                            hipStream_t st);
 double flops_per_attempt();
+
+namespace sg {
+struct SurrogateState;
+void destroy_state(SurrogateState* s);
+}  // namespace sg
 
 }  // namespace grrt_amd
 
@@ -110,11 +116,15 @@ struct GRRTContext {
     grrt_amd::DevBuf<double> batch_rgb;
     grrt_amd::DevBuf<uint8_t> batch_cls;
     grrt_amd::DevBuf<grrt_amd::DevCounters> batch_counters;
+    grrt_amd::DevBuf<GRRTRayRecord> batch_records;   // with keep_records
     bool last_batch = false;            // the last render was a batch (classes come from batch_cls)
     hipGraph_t batch_graph = nullptr;
     hipGraphExec_t batch_exec = nullptr;
     long long batch_rays = 0, batch_pixels = 0;
     int batch_tuning[4] = {0, 0, 0, 0};
+
+    // Learned surrogate (network, optimiser state, datasets), created on demand.
+    grrt_amd::sg::SurrogateState* sg = nullptr;
 
     GRRTStats last{};
     long long last_rays = 0;

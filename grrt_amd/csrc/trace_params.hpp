@@ -110,6 +110,13 @@ struct TraceParams {
     DevCounters* counters;
     const volatile int* cancel;         // host-mapped flag, may be nullptr
 
+    // Listed rays (trace mode 2): fallback ids, then judged ids whose results
+    // are written after the view's n_rays samples; counts in device memory.
+    const long long* fb_list;
+    const long long* judge_list;
+    const unsigned long long* list_counts;   // [0] fallbacks, [1] judged
+    long long judge_cap;
+
     int refill_min;                     // idle lanes that trigger a refill
     int chunk;                          // rays claimed per queue atomic (>= 64)
 

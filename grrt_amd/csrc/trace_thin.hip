@@ -25,6 +25,7 @@
 #include "device_math.hpp"
 #include "kerr_rhs.hpp"
 #include "camera_device.hpp"
+#include "disk_device.hpp"
 #include "star_device.hpp"
 #include "trace_params.hpp"
 
@@ -89,48 +90,13 @@ __device__ __forceinline__ void init_ray(const ViewParams& V, const SampleId& d,
 __device__ __forceinline__ void disk_emission(const ViewParams& V, const double* flux,
                                               const double* s_spec, double rc, double P0,
                                               double P3, Ray& R) {
-    double F = 0.0;
-    if (!(rc <= V.r_in || rc >= V.r_out || V.flux_max <= 0.0)) {
-        const double frac = (rc - V.r_in) / (V.r_out - V.r_in) * (kFluxN - 1);
-        const int idx = static_cast<int>(frac);
-        const double t = frac - idx;
-        F = (idx >= kFluxN - 1) ? flux[kFluxN - 1] : flux[idx] * (1.0 - t) + flux[idx + 1] * t;
-    }
-    // Redshift g = (p.u)_emit / (p.u)_obs for a circular emitter and the
-    // static observer (src/accretion_disk.cpp:113-130).
-    const double w = sqrt(V.M / (rc * rc * rc));
-    const double aw = V.a * w;
-    const double ut_e = 1.0 / sqrt(1.0 - 3.0 * V.M / rc + 2.0 * aw);
-    const double up_e = (w / (1.0 + V.a * w)) * ut_e;
-    const double pu_obs = P0 * V.ut_obs;
-    const double pu_e = P0 * ut_e + P3 * up_e;
-    const double g = (__builtin_fabs(pu_obs) < 1e-30) ? 1.0 : pu_e / pu_obs;
+    const double g = disk_redshift(V, rc, P0, P3);
     if (R.n_cross == 0) {
         R.r_hit = rc;
         R.g_hit = g;
     }
     ++R.n_cross;
-    if (F <= 0.0 || V.flux_max <= 0.0) return;
-    const double T = V.t_peak * sqrt(sqrt(F / V.flux_max));
-    if (T <= 0.0) return;
-    const double Tobs = g * T;
-    if (!(Tobs >= 100.0)) return;       // also drops a NaN temperature
-    // Chromaticity lerp (src/spectrum.cpp:321-330); luminosity (T/1e5)^4.
-    const double Tc = sclamp(Tobs, kSpecTmin, kSpecTmax);
-    const double frac = (Tc - kSpecTmin) / (kSpecTmax - kSpecTmin) * (kSpecN - 1);
-    const int idx = static_cast<int>(frac);
-    const double t = frac - idx;
-    const double ratio = T / kSpecTmax;
-    const double lum = ratio * ratio * ratio * ratio;
-    const double g4 = g * g * g * g;
-    const double wgt = lum * g4;
-#pragma unroll
-    for (int ch = 0; ch < 3; ++ch) {
-        const double cv = (idx >= kSpecN - 1)
-                              ? s_spec[3 * (kSpecN - 1) + ch]
-                              : s_spec[3 * idx + ch] * (1.0 - t) + s_spec[3 * (idx + 1) + ch] * t;
-        R.col[ch] += cv * wgt;
-    }
+    thin_disk_color(V, flux, s_spec, rc, g, R.col);
 }
 
 __device__ __forceinline__ void finish_ray(const TraceParams& P, long long oid, Ray& R, int term) {
@@ -209,17 +175,31 @@ __device__ __forceinline__ int find_view(const ViewParams* views, int n, unsigne
 
 
 // The parameter block lives in device memory; the hot scalars are read once
-// at entry, everything else where it is used.  kBatch: the launch covers
-// several views (each with its own spin, camera and disk), concatenated in
-// queue order; every lane carries its own view's metric constants.
-template <bool kBatch>
+// at entry, everything else where it is used.
+//   kMode 0: one view, queue slot -> sample of the view's window.
+//   kMode 1 (batch): several views (each with its own spin, camera and disk)
+//     concatenated in queue order; every lane carries its own view's metric
+//     constants.
+//   kMode 2 (list): one view, the queue is the listed ray ids (the surrogate's
+//     fallback rays, then its judged rays, whose results go after the frame's
+//     samples); the list lengths are read from device memory.
+template <int kMode>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void trace_thin(const TraceParams* Pg) {
+    constexpr bool kBatch = kMode == 1;
+    constexpr bool kList = kMode == 2;
     __shared__ double s_flux[kBatch ? 1 : kFluxN];
     __shared__ double s_spec[3 * kSpecN];
     const double r_escape = Pg->r_escape, tol = Pg->tol;
     const int max_steps = Pg->max_steps, refill_min = Pg->refill_min, chunk = Pg->chunk;
-    const long long n_queue = Pg->n_rays;
+    unsigned long long n_fb = 0;
+    long long n_queue = Pg->n_rays;
+    if (kList) {
+        n_fb = Pg->list_counts[0];
+        const unsigned long long nj = Pg->list_counts[1];
+        n_queue = static_cast<long long>(n_fb + (nj < static_cast<unsigned long long>(Pg->judge_cap)
+                                                      ? nj : static_cast<unsigned long long>(Pg->judge_cap)));
+    }
     DevCounters* const counters = Pg->counters;
     const int use_disk = kBatch ? 1 : Pg->view.disk;
     if (!kBatch && use_disk) {
@@ -294,12 +274,16 @@ void trace_thin(const TraceParams* Pg) {
                     if (kBatch) vi = find_view(Pg->views, Pg->n_views, mine);
                     const ViewParams& V = kBatch ? Pg->views[vi] : Pg->view;
                     SampleId d;
-                    if (decode_ray(V, static_cast<long long>(mine) - V.q_offset, d)) {
+                    long long q = static_cast<long long>(mine) - V.q_offset;
+                    if (kList) q = mine < n_fb ? Pg->fb_list[mine] : Pg->judge_list[mine - n_fb];
+                    if (decode_ray(V, q, d)) {
                         if (kBatch) {
                             K = KerrK{V.M, V.a, V.a * V.a, 2.0 * V.M};
                             r_term = V.r_term;
                         }
                         oid = V.ray_offset + d.local;
+                        if (kList && mine >= n_fb)
+                            oid = V.n_rays + static_cast<long long>(mine - n_fb);
                         init_ray(V, d, K, R);
                         live = true;
                         const int term = termination(max_steps, r_term, r_escape, R);
@@ -502,9 +486,10 @@ hipError_t launch_reset(DevCounters* c, uint8_t* cls, long long n_rays, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_trace_thin(const TraceParams* P, bool batch, int blocks, int threads, hipStream_t st) {
-    if (batch) hipLaunchKernelGGL(trace_thin<true>, dim3(blocks), dim3(threads), 0, st, P);
-    else hipLaunchKernelGGL(trace_thin<false>, dim3(blocks), dim3(threads), 0, st, P);
+hipError_t launch_trace_thin(const TraceParams* P, int mode, int blocks, int threads, hipStream_t st) {
+    if (mode == 1) hipLaunchKernelGGL(trace_thin<1>, dim3(blocks), dim3(threads), 0, st, P);
+    else if (mode == 2) hipLaunchKernelGGL(trace_thin<2>, dim3(blocks), dim3(threads), 0, st, P);
+    else hipLaunchKernelGGL(trace_thin<0>, dim3(blocks), dim3(threads), 0, st, P);
     return hipGetLastError();
 }
 

@@ -137,6 +137,75 @@ GRRT_EXPORT int grrt_ext_saha(double rho_cgs, double T, double out[5]);
  * alpha_ff, alpha_hminus, alpha_bf [1/cm], B_nu. */
 GRRT_EXPORT int grrt_ext_opacity(double nu, double rho_cgs, double T, double out[6]);
 
+/* ---- Learned surrogate of the ray transfer map ----------------------------
+ * A 5 -> 256 -> 256 -> 256 -> 256 -> 9 MLP (SiLU, BF16 weights, FP32
+ * accumulation on MFMA) from (pixel direction alpha, beta; spin a;
+ * cos theta_obs; log r_obs) to (3 class logits captured/escaped/disk, ln r
+ * and ln g of the first disk crossing, sin/cos of the escape theta and phi),
+ * trained on rays of the context's own FP64 integrator.  The disk extent it
+ * learns is the context's (r_in = ISCO of each spin, r_out = disk_outer). */
+typedef struct {
+    int steps;                /* Adam steps in this call */
+    int batch;                /* rays per step (a multiple of 256) */
+    double lr_max, lr_min;    /* linear warm-up to lr_max, cosine decay to lr_min */
+    int warmup;               /* warm-up steps */
+    int pad;
+    double w_r, w_g, w_dir;   /* loss weights of ln r, ln g, escape direction */
+    uint64_t seed;            /* batch shuffling */
+} GRRTTrainConfig;
+
+typedef struct {
+    int64_t steps;            /* Adam steps taken so far (all calls) */
+    int64_t rays;             /* rows evaluated */
+    double loss, ce;          /* mean total loss and cross-entropy */
+    double accuracy;          /* class accuracy (argmax) */
+    double rmse_lnr, rmse_lng, rmse_dir;  /* on disk rays / escaped rays */
+    double ms;                /* GPU time of the call */
+} GRRTTrainStats;
+
+typedef struct {
+    double infer_ms, trace_ms, judge_ms, total_ms;
+    int64_t rays, fallback_rays, judged_rays, trace_attempts;
+    double judge_class_agreement;     /* judged rays: surrogate class == FP64 class */
+    double judge_median_rel_err;      /* judged lit rays: |L_s - L| / L of summed RGB */
+} GRRTSurrogateStats;
+
+/* Creates (or re-initialises) the network with seeded random weights. */
+GRRT_EXPORT int grrt_ext_surrogate_init(GRRTContext* ctx, uint64_t seed);
+/* Traces n_rays training rays (validation=0; n_rays a power of two) or
+ * validation rays (validation=1) with the FP64 integrator: views of 32x32
+ * pixels, 1 uniform jittered sample each, a ~ U[0, 0.998], theta_obs ~
+ * U[5, 89] deg, r_obs log-uniform in [20, 100], 90 deg field. */
+GRRT_EXPORT int grrt_ext_surrogate_generate(GRRTContext* ctx, int64_t n_rays, uint64_t seed, int validation);
+/* Adam steps over the training set; stats are the running means of the
+ * call's last 10% of steps. */
+GRRT_EXPORT int grrt_ext_surrogate_train(GRRTContext* ctx, const GRRTTrainConfig* cfg, GRRTTrainStats* out);
+/* Metrics on the validation set. */
+GRRT_EXPORT int grrt_ext_surrogate_evaluate(GRRTContext* ctx, GRRTTrainStats* out);
+/* FP32 master parameters (201225 floats): get (set=0) or set (set=1). */
+GRRT_EXPORT int grrt_ext_surrogate_params(GRRTContext* ctx, float* params, int64_t n, int set);
+/* Network outputs y[n][9] for inputs x[n][5] (host memory). */
+GRRT_EXPORT int grrt_ext_surrogate_forward(GRRTContext* ctx, const float* x, int64_t n, float* y);
+/* Loss gradient (201225 floats) and loss at the current parameters for rows
+ * x[n][5] with targets t[n][8] = (class or -1, ln r, ln g, sin th, cos th,
+ * sin ph, cos ph, escaped); n a multiple of 256; weights w[3] = (w_r, w_g,
+ * w_dir). */
+GRRT_EXPORT int grrt_ext_surrogate_gradients(GRRTContext* ctx, const float* x, const float* t, int64_t n,
+                                             const float w[3], float* grad, double* loss);
+/* Copies rows [0, n) of the training (validation=0) or validation set. */
+GRRT_EXPORT int grrt_ext_surrogate_dataset(GRRTContext* ctx, int validation, float* x, float* t, int64_t n);
+/* Surrogate frame of the context's view: every sample is shaded from the
+ * network unless its class entropy exceeds entropy_max (nats) or, for a
+ * predicted disk hit, the network's ln g differs by more than lng_tol from
+ * the redshift that the predicted radius and the ray's conserved p_t, p_phi
+ * fix (lng_tol <= 0: no such check); those samples are traced by the FP64
+ * integrator.  A disk hit is shaded with the exact redshift of its predicted
+ * radius.  judge_fraction of the shaded samples is traced as well and
+ * compared (stats).  out: RGBA float32 as grrt_render (host memory, or
+ * device memory when device_out != 0). */
+GRRT_EXPORT int grrt_ext_render_surrogate(GRRTContext* ctx, float* out, int device_out, double entropy_max,
+                                          double lng_tol, double judge_fraction, GRRTSurrogateStats* stats);
+
 #ifdef __cplusplus
 }
 #endif

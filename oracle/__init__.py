@@ -336,3 +336,61 @@ def vol_kappa_es(vol, rho, T):
 def simplex(perm, x, y, z):
     p = np.ascontiguousarray(perm, np.int32)
     return lib().oracle_simplex(p.ctypes.data_as(C.POINTER(C.c_int)), x, y, z)
+
+
+# ---- Surrogate network (mlp_oracle.cpp) ------------------------------------
+
+MLP_PARAMS = 201225
+
+
+def mlp_lib() -> C.CDLL:
+    if "mlp" in _LIBS:
+        return _LIBS["mlp"]
+    path = os.path.join(_HERE, "lib", "libmlp_oracle.so")
+    if not os.path.exists(path):
+        build()
+    L = C.CDLL(path)
+    _PF = C.POINTER(C.c_float)
+    L.mlp_num_params.restype = C.c_int
+    L.mlp_bf16.argtypes = [C.c_float]
+    L.mlp_bf16.restype = C.c_float
+    L.mlp_forward.argtypes = [_PF, _PF, C.c_long, C.c_int, _PD]
+    L.mlp_gradients.argtypes = [_PF, _PF, _PF, C.c_long, _PF, C.c_int, _PD]
+    L.mlp_gradients.restype = _D
+    L.mlp_adam.argtypes = [_PF, _PF, _PF, _PF, C.c_long, C.c_float, C.c_float, C.c_float, C.c_float, C.c_long]
+    assert L.mlp_num_params() == MLP_PARAMS
+    _LIBS["mlp"] = L
+    return L
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _pf(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def mlp_forward(params, x, emulate=True) -> np.ndarray:
+    p, xx = _f32(params), _f32(x)
+    n = xx.shape[0]
+    y = np.zeros((n, 9), np.float64)
+    mlp_lib().mlp_forward(_pf(p), _pf(xx), n, 1 if emulate else 0, y.ctypes.data_as(_PD))
+    return y
+
+
+def mlp_gradients(params, x, t, w=(1.0, 1.0, 1.0), emulate=True):
+    p, xx, tt, ww = _f32(params), _f32(x), _f32(t), _f32(w)
+    g = np.zeros(MLP_PARAMS, np.float64)
+    loss = mlp_lib().mlp_gradients(_pf(p), _pf(xx), _pf(tt), xx.shape[0], _pf(ww), 1 if emulate else 0,
+                                   g.ctypes.data_as(_PD))
+    return loss, g
+
+
+def mlp_bf16(v: float) -> float:
+    return mlp_lib().mlp_bf16(v)
+
+
+def mlp_adam(w, g, m, v, lr, t, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam step in place on float32 arrays."""
+    mlp_lib().mlp_adam(_pf(w), _pf(_f32(g)), _pf(m), _pf(v), w.size, lr, b1, b2, eps, t)
