@@ -1,5 +1,6 @@
 #!/usr/bin/env python3
-"""Throughput of the GPU geodesic tracer on the Kerr a=0.998 1024^2 4-spp frame.
+"""Throughput of the GPU geodesic tracer on the Kerr a=0.998 1024^2 4-spp frame,
+plain FP64 and with the learned surrogate (FP64 fallback and judge).
 
 One step = one full frame: 1024 x 1024 pixels x 4 samples (seeded jitter,
 seed 1234), thin Novikov–Thorne disk from the ISCO to 20 M, 5000-star
@@ -29,6 +30,11 @@ FP64_PEAK_TFLOPS = 78.6           # MI355X FP64 vector peak
 # formulation: 7 Kerr right-hand sides of ~170 flops plus ~450 flops of stage
 # combinations, error norm and control.
 ALGO_FLOPS_PER_ATTEMPT = 1640.0
+BF16_PEAK_TFLOPS = 2500.0         # MI355X dense BF16 MFMA peak
+# Algorithmic work of one surrogate evaluation: 2 * (5*256 + 3*256*256 + 256*9).
+SURROGATE_FLOPS_PER_RAY = 2.0 * (5 * 256 + 3 * 256 * 256 + 256 * 9)
+# Surrogate gates: class entropy (nats), redshift consistency (ln g), judged share.
+SG_ENTROPY, SG_LNG_TOL, SG_JUDGE = 0.1, 0.005, 0.02
 
 
 def c2_params(grrt):
@@ -65,6 +71,105 @@ def cpu_baseline(target_s: float = 15.0) -> dict:
                       f"oracle built -O3 -ffast-math -mavx2 -mfma, OpenMP guided"}
 
 
+def pixel_flux_error(a, b):
+    """Median relative error of per-pixel summed RGB over pixels lit in
+    either frame, and the share of those pixels within 1%."""
+    import numpy as np
+    la, lb = a[..., :3].sum(-1).astype(np.float64), b[..., :3].sum(-1).astype(np.float64)
+    lit = (la > 0) | (lb > 0)
+    e = np.abs(la[lit] - lb[lit]) / np.where(lb[lit] > 0, lb[lit], np.inf)
+    e = np.where(lb[lit] > 0, e, 1.0)
+    return float(np.median(e)), float(np.mean(e <= 0.01))
+
+
+def surrogate_section(grrt, ctx, plain_value, steps, warmup, sg_steps, c5_frames):
+    """Train the surrogate on rays of the FP64 integrator, then time the
+    surrogate + fallback + judge frame on the same workload and compare it
+    pixel by pixel with the plain FP64 frame."""
+    import numpy as np
+    t0 = time.perf_counter()
+    ctx.surrogate_init(1)
+    ctx.surrogate_generate(1 << 24, seed=7)
+    ctx.surrogate_generate(1 << 16, seed=8, validation=True)
+    gen_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tr = ctx.surrogate_train(steps=sg_steps, batch=16384, lr_max=2e-3, lr_min=2e-5, warmup=200,
+                             w_r=100.0, w_g=100.0, w_dir=1.0, seed=3)
+    train_s = time.perf_counter() - t0
+    ev = ctx.surrogate_evaluate()
+    plain = ctx.render()
+    pcls = ctx.classes()
+    fb, _ = ctx.render_surrogate(SG_ENTROPY, SG_JUDGE, SG_LNG_TOL)
+    scls = ctx.classes()
+    med, within = pixel_flux_error(fb, plain)
+    for _ in range(warmup):
+        ctx.render_surrogate(SG_ENTROPY, SG_JUDGE, SG_LNG_TOL, device=True)
+    stats = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        _, st = ctx.render_surrogate(SG_ENTROPY, SG_JUDGE, SG_LNG_TOL, device=True)
+        stats.append(st)
+    wall = time.perf_counter() - t0
+    rays = W * H * SPP
+    ms = wall / steps * 1e3
+    value = rays / (ms / 1e3) / 1e6
+    infer_ms = sum(s["infer_ms"] for s in stats) / steps
+    trace_ms = sum(s["trace_ms"] for s in stats) / steps
+    achieved = SURROGATE_FLOPS_PER_RAY * rays / (infer_ms / 1e3) / 1e12
+    last = stats[-1]
+    out = {
+        "workload": "the same Kerr a=0.998 1024^2 4-spp frame",
+        "value": round(value, 3), "unit": "Mrays/s", "ms_per_step": round(ms, 3),
+        "speedup_vs_plain": round(value / plain_value, 3),
+        "infer_kernel_ms": round(infer_ms, 3), "trace_kernel_ms": round(trace_ms, 3),
+        "fallback_frac": round(last["fallback_rays"] / last["rays"], 5),
+        "judged_frac": round(last["judged_rays"] / last["rays"], 5),
+        "class_agreement_vs_plain": round(float(np.mean(scls == pcls)), 6),
+        "median_pixel_flux_rel_err_vs_plain": round(med, 5),
+        "pixels_within_1pct": round(within, 4),
+        "judge_class_agreement": round(last["judge_class_agreement"], 6),
+        "judge_median_rel_err": round(last["judge_median_rel_err"], 5),
+        "gates": {"entropy_max_nats": SG_ENTROPY, "lng_tol": SG_LNG_TOL, "judge_fraction": SG_JUDGE},
+        "training": {"rays": 1 << 24, "steps": sg_steps, "batch": 16384, "generate_s": round(gen_s, 2),
+                     "train_s": round(train_s, 2), "val_accuracy": round(ev["accuracy"], 5),
+                     "val_rmse_ln_r": round(ev["rmse_lnr"], 5), "val_rmse_ln_g": round(ev["rmse_lng"], 5)},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / BF16_PEAK_TFLOPS, 4), "traffic": None},
+    }
+    if c5_frames > 0:
+        # The C5 inference frame: 2048^2 x 4 spp of the same scene.
+        p5 = c2_params(grrt)
+        p5.width = p5.height = 2048
+        with grrt.Context(p5) as c5:
+            c5.set_sampling(grrt.SAMPLING_SEEDED, SPP, SEED)
+            c5.surrogate_init(1)
+            c5.surrogate_params(ctx.surrogate_params())
+            plain5 = c5.render()
+            p5cls = c5.classes()
+            c5.render_device()
+            t0 = time.perf_counter()
+            for _ in range(c5_frames):
+                c5.render_device()
+            plain_ms = (time.perf_counter() - t0) / c5_frames * 1e3
+            f5, _ = c5.render_surrogate(SG_ENTROPY, SG_JUDGE, SG_LNG_TOL)
+            s5cls = c5.classes()
+            med5, within5 = pixel_flux_error(f5, plain5)
+            c5.render_surrogate(SG_ENTROPY, SG_JUDGE, SG_LNG_TOL, device=True)
+            t0 = time.perf_counter()
+            for _ in range(c5_frames):
+                _, st5 = c5.render_surrogate(SG_ENTROPY, SG_JUDGE, SG_LNG_TOL, device=True)
+            sg_ms = (time.perf_counter() - t0) / c5_frames * 1e3
+        r5 = 2048 * 2048 * SPP
+        out["c5_2048sq_4spp"] = {
+            "plain_mrays_s": round(r5 / plain_ms / 1e3, 3), "surrogate_mrays_s": round(r5 / sg_ms / 1e3, 3),
+            "speedup": round(plain_ms / sg_ms, 3), "plain_ms": round(plain_ms, 3), "surrogate_ms": round(sg_ms, 3),
+            "class_agreement_vs_plain": round(float(np.mean(s5cls == p5cls)), 6),
+            "median_pixel_flux_rel_err_vs_plain": round(med5, 5),
+            "fallback_frac": round(st5["fallback_rays"] / st5["rays"], 5),
+            "judge_median_rel_err": round(st5["judge_median_rel_err"], 5)}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +177,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--tune", default="", help="block,blocks_per_cu,refill_min,chunk[,tile]")
+    ap.add_argument("--no-surrogate", action="store_true")
+    ap.add_argument("--sg-steps", type=int, default=60000, help="surrogate training steps")
+    ap.add_argument("--c5-frames", type=int, default=2, help="2048^2 frames timed per form (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
@@ -134,6 +242,9 @@ def main() -> int:
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None},
     }
+    if not args.no_surrogate:
+        out["surrogate"] = surrogate_section(grrt, ctx, value, args.steps, args.warmup, args.sg_steps,
+                                             args.c5_frames)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     ctx.close()

@@ -165,6 +165,10 @@ API = [
     ("grrt_ext_autotune", C.c_int, [_P, C.POINTER(GRRTView), C.c_int, C.POINTER(C.c_int),
                                     C.POINTER(C.c_double)]),
     ("grrt_ext_set_precision", C.c_int, [_P, C.c_int]),
+    ("grrt_ext_render_spectral_to_fits_f32", C.c_int, [_P, C.c_char_p, C.c_int, C.c_int, PROGRESS_FN, _P]),
+    ("grrt_ext_write_fits_cube", C.c_int, [C.c_char_p, _P, C.c_int, C.c_int, C.c_int, C.c_int,
+                                           C.POINTER(C.c_double), C.POINTER(GRRTParams)]),
+    ("grrt_ext_write_hdr", C.c_int, [C.c_char_p, C.POINTER(C.c_float), C.c_int, C.c_int, C.c_int]),
     ("grrt_ext_surrogate_init", C.c_int, [_P, C.c_uint64]),
     ("grrt_ext_surrogate_generate", C.c_int, [_P, C.c_int64, C.c_uint64, C.c_int]),
     ("grrt_ext_surrogate_train", C.c_int, [_P, C.POINTER(GRRTTrainConfig), C.POINTER(GRRTTrainStats)]),
@@ -344,6 +348,14 @@ class Context:
             o += n
         return frames
 
+    def render_spectral_to_fits(self, path: str, f32: bool = False, progress=None) -> int:
+        """Stream the spectral cube to a FITS file (BITPIX -64, or -32 with
+        f32); returns 0, or 1 if cancelled."""
+        cb = PROGRESS_FN(lambda f, _ud: progress(f)) if progress else PROGRESS_FN(0)
+        fn = self._L.grrt_ext_render_spectral_to_fits_f32 if f32 else self._L.grrt_render_spectral_to_fits_cb
+        return self._check(fn(self._h, path.encode(), self.params.width, self.params.height, cb, None),
+                           "render_spectral_to_fits")
+
     # ---- learned surrogate ----
     def surrogate_init(self, seed: int = 0):
         self._check(self._L.grrt_ext_surrogate_init(self._h, seed), "surrogate_init")
@@ -398,10 +410,16 @@ class Context:
                                                         t.ctypes.data, n), "surrogate_dataset")
         return x, t
 
-    def render_surrogate(self, entropy_max=0.1, judge_fraction=0.02, lng_tol=0.0, d_ptr: int | None = None):
-        """Surrogate frame with FP64 fallback and judge; returns (frame or
-        None when rendering into device memory, stats dict)."""
+    def render_surrogate(self, entropy_max=0.1, judge_fraction=0.02, lng_tol=0.0, d_ptr: int | None = None,
+                         device: bool = False):
+        """Surrogate frame with FP64 fallback and judge; returns (frame, stats
+        dict).  d_ptr: render into that device buffer; device=True: keep the
+        frame in the context's device memory (frame None)."""
         st = GRRTSurrogateStats()
+        if device:
+            self._check(self._L.grrt_ext_render_surrogate(self._h, None, 1, entropy_max, lng_tol,
+                                                           judge_fraction, C.byref(st)), "render_surrogate")
+            return None, _as_dict(st)
         if d_ptr is not None:
             self._check(self._L.grrt_ext_render_surrogate(self._h, d_ptr, 1, entropy_max, lng_tol,
                                                            judge_fraction, C.byref(st)), "render_surrogate")
@@ -509,3 +527,26 @@ def tonemap(fb: np.ndarray) -> np.ndarray:
     h, w = out.shape[:2]
     lib().grrt_tonemap(out.ctypes.data_as(C.POINTER(C.c_float)), w, h)
     return out
+
+
+def write_fits_cube(path: str, cube: np.ndarray, freqs=None, params: GRRTParams | None = None,
+                    bitpix: int = -64) -> None:
+    """Write a (height, width, bins) cube as FITS (grrt_ext_write_fits_cube)."""
+    dt = np.float64 if bitpix == -64 else np.float32
+    c = np.ascontiguousarray(cube, dt)
+    h, w, nb = c.shape
+    f = None if freqs is None else (C.c_double * nb)(*[float(v) for v in freqs])
+    rc = lib().grrt_ext_write_fits_cube(path.encode(), c.ctypes.data, bitpix, w, h, nb, f,
+                                        C.byref(params) if params is not None else None)
+    if rc != 0:
+        raise RuntimeError(lib().grrt_last_error().decode())
+
+
+def write_hdr(path: str, rgba: np.ndarray, normalised: bool = False) -> None:
+    """Write an (height, width, 4) float32 frame as a Radiance RGBE .hdr."""
+    fb = np.ascontiguousarray(rgba, np.float32)
+    h, w, _ = fb.shape
+    rc = lib().grrt_ext_write_hdr(path.encode(), fb.ctypes.data_as(C.POINTER(C.c_float)), w, h,
+                                  1 if normalised else 0)
+    if rc != 0:
+        raise RuntimeError(lib().grrt_last_error().decode())

@@ -14,6 +14,7 @@
 #include <thread>
 
 #include "api_internal.hpp"
+#include "writers_host.hpp"
 #include "context.hpp"
 #include "grrt/api.h"
 #include "grrt_ext.h"
@@ -374,6 +375,75 @@ void tonemap_host(float* fb, int w, int h) {
 
 using namespace grrt_api;
 
+namespace grrt_api {
+
+// FITS metadata with the ABI's zero-means-default rules
+// (src/api.cpp:296-302 of the reference).
+grrt_amd::FitsMeta fits_meta(const GRRTParams& p) {
+    grrt_amd::FitsMeta m;
+    m.spin = p.spin;
+    m.mass = p.mass > 0.0 ? p.mass : 1.0;
+    m.observer_r = p.observer_r > 0.0 ? p.observer_r : 50.0;
+    m.observer_theta = p.observer_theta > 0.0 ? p.observer_theta : 1.396;
+    m.fov = p.fov > 0.0 ? p.fov : 1.047;
+    m.samples_per_pixel = p.samples_per_pixel > 0 ? p.samples_per_pixel : 1;
+    return m;
+}
+
+// Spectral cube streamed to a FITS file in bands of rows: each band is
+// rendered on the GPU into device memory, copied back and written row by
+// row at its place in the pre-extended file (BITPIX -64, or -32 for the
+// float32 cube).  Returns 0, 1 if cancelled (unrendered rows stay zero), or
+// -1 with the reference's argument errors.
+int spectral_to_fits(GRRTContext* ctx, const char* path, int width, int height, bool f32,
+                     grrt_progress_fn progress, void* ud) {
+    if (ctx->frequency_bins.empty()) {
+        ctx->error_msg = "No frequency bins set -- call grrt_set_frequency_bins first";
+        return -1;
+    }
+    if (!path || path[0] == '\0') {
+        ctx->error_msg = "grrt_render_spectral_to_fits_cb: output_path is null or empty";
+        return -1;
+    }
+    if (width <= 0 || height <= 0) throw HipFailure("grrt_render_spectral_to_fits_cb: empty image");
+    const int nb = static_cast<int>(ctx->frequency_bins.size());
+    grrt_amd::FitsCubeWriter out(path, width, height, nb, ctx->frequency_bins, fits_meta(ctx->params), f32 ? -32 : -64);
+    ctx->progress.store(0.0f);
+    int rc = 0;
+    if (ctx->res.volumetric) {
+        const size_t es = f32 ? sizeof(float) : sizeof(double);
+        // About 2^18 spectral values per band.
+        const int band = std::clamp((1 << 18) / std::max(width * nb, 1), 1, height);
+        std::vector<unsigned char> host(static_cast<size_t>(width) * band * nb * es);
+        check(ctx->spec_out.ensure(host.size()), "hipMalloc(spectral band)");
+        for (int y0 = 0; y0 < height && rc == 0; y0 += band) {
+            const int bh = std::min(band, height - y0);
+            rc = render_window(ctx, 0, y0, width, bh, ctx->spec_out.ptr, nullptr, nullptr,
+                               f32 ? Output::kSpectralF32 : Output::kSpectralF64);
+            if (rc != 0) break;
+            const size_t row_bytes = static_cast<size_t>(width) * nb * es;
+            check(hipMemcpy(host.data(), ctx->spec_out.ptr, row_bytes * bh, hipMemcpyDeviceToHost), "copy spectra");
+            for (int j = 0; j < bh; ++j) {
+                if (f32) out.write_row(y0 + j, reinterpret_cast<const float*>(host.data() + row_bytes * j));
+                else out.write_row(y0 + j, reinterpret_cast<const double*>(host.data() + row_bytes * j));
+            }
+            const float f = static_cast<float>(y0 + bh) / static_cast<float>(height);
+            ctx->progress.store(f);
+            if (progress) progress(f, ud);
+        }
+    } else if (progress) {
+        // No volumetric disk: every spectrum is zero (src/geodesic_tracer.cpp:503-505),
+        // which the pre-extended file already holds.
+        progress(1.0f, ud);
+    }
+    out.finish();
+    ctx->progress.store(1.0f);
+    return rc;
+}
+
+}  // namespace grrt_api
+
+
 extern "C" {
 
 GRRTContext* grrt_create(const GRRTParams* params) {
@@ -567,21 +637,7 @@ int grrt_render_spectral_cb(GRRTContext* ctx, double* spectral_buffer, int width
 
 int grrt_render_spectral_to_fits_cb(GRRTContext* ctx, const char* output_path, int width, int height,
                                     grrt_progress_fn progress, void* user_data) {
-    (void)width;
-    (void)height;
-    (void)progress;
-    (void)user_data;
-    return guarded(ctx, [&] {
-        if (ctx->frequency_bins.empty()) {
-            ctx->error_msg = "No frequency bins set -- call grrt_set_frequency_bins first";
-            return -1;
-        }
-        if (!output_path || output_path[0] == '\0') {
-            ctx->error_msg = "grrt_render_spectral_to_fits_cb: output_path is null or empty";
-            return -1;
-        }
-        throw HipFailure("the FITS writer is not available in this build of the GPU backend");
-    });
+    return guarded(ctx, [&] { return spectral_to_fits(ctx, output_path, width, height, false, progress, user_data); });
 }
 
 // ------------------------------------------------------------ extensions ---
@@ -960,6 +1016,48 @@ int grrt_ext_autotune(GRRTContext* ctx, const GRRTView* views, int n, int best[4
         if (best_ms) *best_ms = best_t;
         return 0;
     });
+}
+
+}  // extern "C"
+
+extern "C" {
+
+int grrt_ext_render_spectral_to_fits_f32(GRRTContext* ctx, const char* path, int width, int height,
+                                         grrt_progress_fn progress, void* user_data) {
+    return guarded(ctx, [&] { return spectral_to_fits(ctx, path, width, height, true, progress, user_data); });
+}
+
+int grrt_ext_write_fits_cube(const char* path, const void* cube, int bitpix, int width, int height, int num_bins,
+                             const double* frequencies_hz, const GRRTParams* meta) {
+    try {
+        if (!path || !cube || (bitpix != -64 && bitpix != -32)) throw std::runtime_error("invalid FITS cube arguments");
+        std::vector<double> f;
+        if (frequencies_hz) f.assign(frequencies_hz, frequencies_hz + std::max(num_bins, 0));
+        GRRTParams zero{};
+        grrt_amd::FitsCubeWriter out(path, width, height, num_bins, f, fits_meta(meta ? *meta : zero), bitpix);
+        const size_t row = static_cast<size_t>(width) * num_bins;
+        for (int j = 0; j < height; ++j) {
+            if (bitpix == -64) out.write_row(j, static_cast<const double*>(cube) + row * j);
+            else out.write_row(j, static_cast<const float*>(cube) + row * j);
+        }
+        out.finish();
+        return 0;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return -1;
+    }
+}
+
+int grrt_ext_write_hdr(const char* path, const float* rgba, int width, int height, int normalised) {
+    try {
+        if (!path) throw std::runtime_error("null path");
+        const float scale = normalised ? grrt_amd::hdr_exposure_scale(rgba, width, height) : 1.0f;
+        grrt_amd::write_rgbe(path, rgba, width, height, scale);
+        return 0;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return -1;
+    }
 }
 
 }  // extern "C"

@@ -264,22 +264,23 @@ __device__ __forceinline__ long long wave_append(bool want, unsigned long long* 
     return want ? static_cast<long long>(base + lanes_below(mask)) : -1;
 }
 
-// Surrogate frame pass.  Persistent blocks of 4 waves (two blocks per CU,
-// so one block's SiLU work overlaps the other's MFMAs); each wave owns 32
-// rays of a 128-ray tile.  The hidden and output weights stream through a
-// double-buffered 16 KB LDS window, one 32-neuron block (16 k-steps) at a
-// time, shared by the block's four waves; biases and the layer-0 weights
-// stay in LDS.  Per ray: inputs from the view's sample position, the
-// network, then either shading from the prediction or a place in the
-// fallback list; a hashed fraction of the shaded rays is listed for the
-// FP64 judge.
+// Surrogate frame pass.  Persistent blocks of 4 waves (two blocks per CU);
+// each wave owns 32 rays of a 128-ray tile.  Hidden-layer weights stream
+// through a double-buffered 16 KB LDS window, one 32-neuron block (16
+// k-steps) at a time, shared by the block's four waves; the layer-0 and
+// output-layer weights and all biases stay in LDS.  Every finished 32-neuron
+// block goes through SiLU while the MFMAs of the next block run (the two
+// are independent, so the SIMD overlaps them), and the output layer is
+// accumulated block by block inside the last hidden layer.  Per ray: inputs
+// from the view's sample position, the network, then either shading from
+// the prediction or a place in the fallback list; a hashed fraction of the
+// shaded rays is listed for the FP64 judge.
 constexpr int kInferTile = 128;
 constexpr int kChunkVec = 16 * 64;          // uint4 (8 BF16) per 16 KB weight window
-constexpr int kChunks = 3 * 8 + 1;          // hidden (layer, 32-neuron block) windows + the output layer
+constexpr int kChunks = 3 * 8;              // hidden (layer, 32-neuron block) windows
 
 __device__ __forceinline__ const uint4* chunk_src(const MlpArgs& A, int c) {
-    return c < 24 ? reinterpret_cast<const uint4*>(A.packH + static_cast<long long>(c) * kChunkVec * 8)
-                  : reinterpret_cast<const uint4*>(A.pack4);
+    return reinterpret_cast<const uint4*>(A.packH + static_cast<long long>(c) * kChunkVec * 8);
 }
 
 // Copy a 16 KB weight window global -> LDS with the direct-to-LDS loads
@@ -287,6 +288,10 @@ __device__ __forceinline__ const uint4* chunk_src(const MlpArgs& A, int c) {
 // l's 16 bytes landing at row base + 16 l.  Completion is awaited with
 // s_waitcnt before the block barrier that publishes the window.
 __device__ __forceinline__ void load_window(const uint4* src, uint4* dst) {
+    // The window's base address stays an opaque scalar here, so the
+    // compiler does not precompute (and spill) every window's per-lane
+    // addresses ahead of the network.
+    asm volatile("" : "+s"(src));
     const int tid = threadIdx.x, wbase = tid & ~63;
 #pragma unroll
     for (int k = 0; k < 4; ++k)
@@ -294,9 +299,55 @@ __device__ __forceinline__ void load_window(const uint4* src, uint4* dst) {
                                          (__attribute__((address_space(3))) void*)(dst + 256 * k + wbase), 16, 0, 0);
 }
 
+// Scheduling pattern for one weight window: each MFMA is followed by the LDS
+// read of a later k-step's weights and a share of the previous block's SiLU
+// (VALU), so LDS latency and SiLU work hide under the MFMA chain.
+template <int valu_per_mfma>
+__device__ __forceinline__ void interleave_window() {
+    __builtin_amdgcn_sched_group_barrier(0x100, 7, 0);       // DS read: 4 bias + 3 weights (prefetch distance 3)
+#pragma unroll
+    for (int s = 0; s < 13; ++s) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // MFMA
+        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);   // DS read
+        __builtin_amdgcn_sched_group_barrier(0x002, valu_per_mfma, 0);   // VALU
+    }
+#pragma unroll
+    for (int s = 0; s < 3; ++s) {
+        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        __builtin_amdgcn_sched_group_barrier(0x002, valu_per_mfma, 0);
+    }
+}
+
+// One 32-neuron block of a layer from a weight window: the bias, then the
+// 16 k-step MFMA chain with the window's reads issued three k-steps ahead.
+__device__ __forceinline__ f32x16 window_block(const bf16x8* W, const bf16x8 (&in)[16], const float* bias, int mt,
+                                               int g, int lane) {
+    f32x16 a = bias_tile(bias, mt, g, kHid);
+    const bf16x8* Wl = W + lane;
+    bf16x8 w0 = Wl[0], w1 = Wl[64], w2 = Wl[128], w3;
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+        if (s + 3 < 16) w3 = Wl[(s + 3) * 64];
+        a = mfma(w0, in[s], a);
+        w0 = w1;
+        w1 = w2;
+        w2 = w3;
+    }
+    return a;
+}
+
+// SiLU of one 32-neuron accumulator block into B-operand slots 2mt, 2mt+1.
+__device__ __forceinline__ void silu_block(const f32x16& a, bf16x8& lo_slot, bf16x8& hi_slot) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) lo_slot[j] = static_cast<__bf16>(silu(a[j]));
+#pragma unroll
+    for (int j = 0; j < 8; ++j) hi_slot[j] = static_cast<__bf16>(silu(a[8 + j]));
+}
+
 __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
     __shared__ uint4 wbuf[2][kChunkVec];
     __shared__ uint4 s_pack0[kPack0 / 8];
+    __shared__ uint4 s_pack4[kPack4 / 8];
     __shared__ float s_bias[4 * kHid + 16];
     __shared__ float outs[4][32][kOut];
     __shared__ double s_flux[kFluxN];
@@ -306,11 +357,11 @@ __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, g = lane >> 5, c = lane & 31;
     for (int i = tid; i < kFluxN; i += 256) s_flux[i] = V.flux_lut[i];
     for (int i = tid; i < kPack0 / 8; i += 256) s_pack0[i] = reinterpret_cast<const uint4*>(A.pack0)[i];
+    for (int i = tid; i < kPack4 / 8; i += 256) s_pack4[i] = reinterpret_cast<const uint4*>(A.pack4)[i];
     for (int i = tid; i < 4 * kHid; i += 256) s_bias[i] = A.params[bias_offset(i / kHid) + i % kHid];
     if (tid < 16) s_bias[4 * kHid + tid] = tid < kOut ? A.params[oB4 + tid] : 0.0f;
     const long long n = A.n;
     const long long n_tiles = (n + kInferTile - 1) / kInferTile;
-    const double a_star = V.a / V.M, cth = cos(V.th_obs), r_star = V.r_obs / V.M;
     if (static_cast<long long>(blockIdx.x) < n_tiles) load_window(chunk_src(A, 0), wbuf[0]);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -333,46 +384,87 @@ __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
             const double beta = (py - V.H / 2.0) * V.fov / V.W;
             features(alpha, beta, V.a / V.M, cos(V.th_obs), V.r_obs / V.M, x[0]);
         }
-        f32x16 acc[1][8];
-        bf16x8 act[1][16];
-        input_layer<1>(x, reinterpret_cast<const __bf16*>(s_pack0), s_bias, acc);
-        finish_hidden<1, false>(A, 0, ray0, acc, act);
+        bf16x8 actA[16], actB[16];
+        // Layer 0: one k-step per 32-neuron block.
+        {
+            bf16x8 in0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int f = g * 8 + j;
+                float v = 0.0f;
+                if (f < 2 * kIn) {
+                    const float xi = x[0][f >> 1];
+                    const float hi = bf16r(xi);
+                    v = (f & 1) ? xi - hi : hi;
+                }
+                in0[j] = static_cast<__bf16>(v);
+            }
+            const bf16x8* w0 = reinterpret_cast<const bf16x8*>(s_pack0);
+#pragma unroll
+            for (int mt = 0; mt < 8; ++mt) {
+                const f32x16 a = mfma(w0[mt * 64 + lane], in0, bias_tile(s_bias, mt, g, kHid));
+                silu_block(a, actA[2 * mt], actA[2 * mt + 1]);
+            }
+        }
         // One 16 KB weight window: start the copy of the next window into
-        // the other buffer, run the MFMAs on this one, then wait for the copy
-        // and publish it with the block barrier.
-        auto run_chunk = [&](int cidx, auto&& compute) {
+        // the other buffer, run this window's block (and the SiLU of the
+        // previous block), then wait for the copy and publish it.
+        auto window = [&](int cidx, auto&& body) {
             const bool has_next = cidx + 1 < kChunks || more_tiles;
             if (has_next) load_window(chunk_src(A, cidx + 1 < kChunks ? cidx + 1 : 0), wbuf[buf ^ 1]);
-            compute(static_cast<const uint4*>(wbuf[buf]));
+            body(reinterpret_cast<const bf16x8*>(wbuf[buf]));
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
             buf ^= 1;
         };
-#pragma unroll 1
-        for (int layer = 1; layer <= 3; ++layer) {
+        auto hidden = [&](int layer, const bf16x8 (&in)[16], bf16x8 (&out)[16]) {
+            f32x16 prev;
 #pragma unroll
             for (int mt = 0; mt < 8; ++mt) {
-                run_chunk((layer - 1) * 8 + mt, [&](const uint4* W) {
-                    f32x16 a = bias_tile(s_bias + layer * kHid, mt, g, kHid);
-#pragma unroll
-                    for (int s = 0; s < 16; ++s) a = mfma(__builtin_bit_cast(bf16x8, W[s * 64 + lane]), act[0][s], a);
-                    acc[0][mt] = a;
+                window((layer - 1) * 8 + mt, [&](const bf16x8* W) {
+                    const f32x16 a = window_block(W, in, s_bias + layer * kHid, mt, g, lane);
+                    if (mt > 0) silu_block(prev, out[2 * mt - 2], out[2 * mt - 1]);
+                    prev = a;
+                    interleave_window<8>();
                 });
             }
-            if (layer < 3) finish_hidden<1, false>(A, layer, ray0, acc, act);
-        }
-        finish_last_hidden<1, false>(A, ray0, acc);
+            silu_block(prev, out[14], out[15]);
+        };
+        hidden(1, actA, actB);
+        hidden(2, actB, actA);
+        // Layer 3 with the output layer folded in: block mt's FP32
+        // activations enter output k-steps 2mt, 2mt+1 as BF16 hi + lo.
         f32x16 o = bias_tile(s_bias + 4 * kHid, 0, g, kOut);
-        run_chunk(kChunks - 1, [&](const uint4* W) {
+        {
+            const bf16x8* w4 = reinterpret_cast<const bf16x8*>(s_pack4);
+            auto out_block = [&](const f32x16& a, int mt) {
 #pragma unroll
-            for (int s = 0; s < 16; ++s) {
-                const bf16x8 w = __builtin_bit_cast(bf16x8, W[s * 64 + lane]);
-                bf16x8 hi, lo;
-                hilo_operand<1>(acc, 0, s, hi, lo);
-                o = mfma(w, hi, o);
-                o = mfma(w, lo, o);
+                for (int q = 0; q < 2; ++q) {
+                    bf16x8 hi, lo;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const float h = silu(a[q * 8 + j]);
+                        const __bf16 hh = static_cast<__bf16>(h);
+                        hi[j] = hh;
+                        lo[j] = static_cast<__bf16>(h - static_cast<float>(hh));
+                    }
+                    const bf16x8 w = w4[(2 * mt + q) * 64 + lane];
+                    o = mfma(w, hi, o);
+                    o = mfma(w, lo, o);
+                }
+            };
+            f32x16 prev;
+#pragma unroll
+            for (int mt = 0; mt < 8; ++mt) {
+                window(16 + mt, [&](const bf16x8* W) {
+                    const f32x16 a = window_block(W, actA, s_bias + 3 * kHid, mt, g, lane);
+                    if (mt > 0) out_block(prev, mt - 1);
+                    prev = a;
+                    interleave_window<10>();
+                });
             }
-        });
+            out_block(prev, 7);
+        }
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             const int m = acc_row(g, v);

@@ -439,7 +439,7 @@ int grrt_ext_render_surrogate(GRRTContext* ctx, float* out, int device_out, doub
         SurrogateState& S = state(ctx);
         const Resolved& r = ctx->res;
         if (r.volumetric) throw HipFailure("the surrogate covers the thin disk");
-        if (!out) throw HipFailure("null output");
+        if (!out && !device_out) throw HipFailure("null output");
         const auto t0 = std::chrono::steady_clock::now();
         const hipStream_t st = ctx->stream;
         int grid = 1;
@@ -517,9 +517,10 @@ int grrt_ext_render_surrogate(GRRTContext* ctx, float* out, int device_out, doub
         const long long nj = static_cast<long long>(std::min<unsigned long long>(cnt[1], static_cast<unsigned long long>(cap)));
         std::vector<float> errs(static_cast<size_t>(nj));
         if (nj) check(hipMemcpy(errs.data(), S.judge_err.ptr, sizeof(float) * nj, hipMemcpyDeviceToHost), "judge");
-        check(hipMemcpy(out, ctx->frame.ptr, sizeof(float) * 4 * npix,
-                        device_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost),
-              "copy frame");
+        if (out)
+            check(hipMemcpy(out, ctx->frame.ptr, sizeof(float) * 4 * npix,
+                            device_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost),
+                  "copy frame");
         float ms[2] = {0.0f, 0.0f};
         check(hipEventElapsedTime(&ms[0], ctx->ev[0], ctx->ev[1]), "event time");
         check(hipEventElapsedTime(&ms[1], ctx->ev[1], ctx->ev[2]), "event time");

@@ -1,0 +1,273 @@
+// FITS cube and Radiance RGBE writers (host C++).
+#include "writers_host.hpp"
+
+#include <fcntl.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+
+namespace grrt_amd {
+namespace {
+
+constexpr int kCard = 80;
+constexpr int kBlock = 2880;
+
+// One header card: keyword in columns 1-8, "= ", the value right-justified
+// in columns 11-30, then " / comment"; blank-padded (or cut) to 80 columns.
+std::string card(const std::string& keyword, std::string value, const std::string& comment) {
+    std::string s = keyword;
+    s.resize(8, ' ');
+    s += "= ";
+    if (value.size() < 20) value.insert(0, 20 - value.size(), ' ');
+    s += value;
+    if (!comment.empty()) s += " / " + comment;
+    s.resize(kCard, ' ');
+    return s;
+}
+
+std::string fmt_double(double v) {
+    char b[64];
+    std::snprintf(b, sizeof b, "%.12E", v);
+    return b;
+}
+
+std::string fmt_int(long long v) { return std::to_string(v); }
+
+// Quoted string value: at least 8 characters inside the quotes, at most 18.
+std::string fmt_string(std::string v) {
+    if (v.size() < 8) v.resize(8, ' ');
+    if (v.size() > 18) v.resize(18);
+    return "'" + v + "'";
+}
+
+bool uniform_linear(const std::vector<double>& f) {
+    if (f.size() < 2) return true;
+    const double step = f[1] - f[0];
+    if (step == 0.0) return false;
+    for (size_t i = 2; i < f.size(); ++i)
+        if (std::fabs(((f[i] - f[i - 1]) - step) / step) > 1e-4) return false;
+    return true;
+}
+
+bool uniform_log(const std::vector<double>& f) {
+    if (f.size() < 2) return true;
+    if (f[0] <= 0.0 || f[1] <= 0.0) return false;
+    const double step = std::log10(f[1]) - std::log10(f[0]);
+    if (step == 0.0) return false;
+    for (size_t i = 2; i < f.size(); ++i) {
+        if (f[i] <= 0.0) return false;
+        if (std::fabs(((std::log10(f[i]) - std::log10(f[i - 1])) - step) / step) > 1e-4) return false;
+    }
+    return true;
+}
+
+std::string freq_key(int k) {
+    char b[16];
+    std::snprintf(b, sizeof b, "FREQ%04d", k);
+    return b;
+}
+
+void put_be(unsigned char* dst, double v) {
+    uint64_t u;
+    std::memcpy(&u, &v, 8);
+    for (int b = 0; b < 8; ++b) dst[b] = static_cast<unsigned char>(u >> (56 - 8 * b));
+}
+
+void put_be(unsigned char* dst, float v) {
+    uint32_t u;
+    std::memcpy(&u, &v, 4);
+    for (int b = 0; b < 4; ++b) dst[b] = static_cast<unsigned char>(u >> (24 - 8 * b));
+}
+
+void write_all(int fd, const void* p, size_t n, long long off, const std::string& path) {
+    const char* c = static_cast<const char*>(p);
+    while (n > 0) {
+        const ssize_t w = ::pwrite(fd, c, n, static_cast<off_t>(off));
+        if (w < 0) {
+            if (errno == EINTR) continue;
+            throw std::runtime_error("write error on '" + path + "': " + std::strerror(errno));
+        }
+        c += w;
+        off += w;
+        n -= static_cast<size_t>(w);
+    }
+}
+
+}  // namespace
+
+std::vector<std::string> fits_header_cards(int width, int height, int num_bins, const std::vector<double>& freqs,
+                                           const FitsMeta& meta, int bitpix) {
+    std::vector<std::string> c;
+    c.push_back(card("SIMPLE", "T", "conforms to FITS standard"));
+    c.push_back(card("BITPIX", fmt_int(bitpix), bitpix == -64 ? "IEEE 754 double precision"
+                                                              : "IEEE 754 single precision"));
+    c.push_back(card("NAXIS", "3", "number of array dimensions"));
+    c.push_back(card("NAXIS1", fmt_int(width), "pixels along RA axis"));
+    c.push_back(card("NAXIS2", fmt_int(height), "pixels along Dec axis"));
+    c.push_back(card("NAXIS3", fmt_int(num_bins), "number of frequency bins"));
+    c.push_back(card("BUNIT", fmt_string("erg/s/cm2/Hz/sr"), "specific intensity"));
+    if (!freqs.empty()) {
+        const int nk = std::min<int>(num_bins, static_cast<int>(freqs.size()));
+        if (freqs.size() >= 2 && uniform_linear(freqs)) {
+            c.push_back(card("CTYPE3", fmt_string("FREQ"), "frequency"));
+            c.push_back(card("CUNIT3", fmt_string("Hz"), "frequency unit"));
+            c.push_back(card("CRPIX3", fmt_double(1.0), "reference pixel"));
+            c.push_back(card("CRVAL3", fmt_double(freqs[0]), "frequency at ref pixel (Hz)"));
+            c.push_back(card("CDELT3", fmt_double(freqs[1] - freqs[0]), "frequency step (Hz)"));
+        } else {
+            const bool lg = freqs.size() >= 2 && uniform_log(freqs);
+            if (lg) {
+                c.push_back(card("CTYPE3", fmt_string("FREQ-LOG"), "log10(frequency/Hz)"));
+                c.push_back(card("CUNIT3", fmt_string("log(Hz)"), "log10 of frequency in Hz"));
+                c.push_back(card("CRPIX3", fmt_double(1.0), "reference pixel"));
+                c.push_back(card("CRVAL3", fmt_double(std::log10(freqs[0])), "log10(freq) at ref pixel"));
+                c.push_back(card("CDELT3", fmt_double(std::log10(freqs[1]) - std::log10(freqs[0])),
+                                 "log10(freq) step per pixel"));
+            } else {
+                c.push_back(card("CTYPE3", fmt_string("FREQ-TAB"), "frequency (see FREQnnnn keys)"));
+                c.push_back(card("CUNIT3", fmt_string("Hz"), "frequency unit"));
+                c.push_back(card("CRPIX3", fmt_double(1.0), "reference pixel"));
+                c.push_back(card("CRVAL3", fmt_double(1.0), "slice index"));
+                c.push_back(card("CDELT3", fmt_double(1.0), "slice index step"));
+            }
+            for (int k = 0; k < nk; ++k)
+                c.push_back(card(freq_key(k + 1), fmt_double(freqs[k]), "freq bin " + std::to_string(k + 1) + " (Hz)"));
+        }
+    }
+    c.push_back(card("SPIN", fmt_double(meta.spin), "Kerr spin parameter a/M"));
+    c.push_back(card("MASS", fmt_double(meta.mass), "black hole mass (geometrized units)"));
+    c.push_back(card("OBS_R", fmt_double(meta.observer_r), "observer radial coordinate (M)"));
+    c.push_back(card("OBS_TH", fmt_double(meta.observer_theta), "observer polar angle (rad)"));
+    c.push_back(card("FOV", fmt_double(meta.fov), "camera field of view (rad)"));
+    c.push_back(card("SPP", fmt_int(meta.samples_per_pixel), "samples per pixel (anti-aliasing)"));
+    c.push_back(card("ORIGIN", fmt_string("grrt"), "GR ray tracer"));
+    std::string end = "END";
+    end.resize(kCard, ' ');
+    c.push_back(end);
+    return c;
+}
+
+FitsCubeWriter::FitsCubeWriter(const std::string& path, int width, int height, int num_bins,
+                               const std::vector<double>& freqs, const FitsMeta& meta, int bitpix)
+    : width_(width), height_(height), bins_(num_bins), bitpix_(bitpix), path_(path) {
+    if (width <= 0 || height <= 0 || num_bins <= 0) throw std::runtime_error("FITS cube with an empty axis");
+    if (bitpix != -64 && bitpix != -32) throw std::runtime_error("FITS BITPIX must be -64 or -32");
+    fd_ = ::open(path.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
+    if (fd_ < 0) throw std::runtime_error("cannot open '" + path + "' for writing: " + std::strerror(errno));
+    const auto cards = fits_header_cards(width, height, num_bins, freqs, meta, bitpix);
+    std::string header;
+    for (const auto& c : cards) header += c;
+    header.resize((header.size() + kBlock - 1) / kBlock * kBlock, ' ');
+    write_all(fd_, header.data(), header.size(), 0, path_);
+    data_start_ = static_cast<long long>(header.size());
+    const long long bytes = static_cast<long long>(width) * height * num_bins * (bitpix == -64 ? 8 : 4);
+    const long long padded = (bytes + kBlock - 1) / kBlock * kBlock;
+    if (::ftruncate(fd_, static_cast<off_t>(data_start_ + padded)) != 0)
+        throw std::runtime_error("cannot extend '" + path + "': " + std::strerror(errno));
+    line_.resize(static_cast<size_t>(width) * (bitpix == -64 ? 8 : 4));
+}
+
+FitsCubeWriter::~FitsCubeWriter() {
+    if (fd_ >= 0) ::close(fd_);
+}
+
+template <class T>
+void FitsCubeWriter::put_row(int j, const T* row) {
+    if (j < 0 || j >= height_) throw std::runtime_error("FITS row out of range");
+    const int fits_row = height_ - 1 - j;        // FITS pixel (1,1) is bottom-left
+    const size_t es = bitpix_ == -64 ? 8 : 4;
+    for (int k = 0; k < bins_; ++k) {
+        for (int i = 0; i < width_; ++i) {
+            const T v = row[static_cast<size_t>(i) * bins_ + k];
+            if (es == 8) put_be(&line_[i * es], static_cast<double>(v));
+            else put_be(&line_[i * es], static_cast<float>(v));
+        }
+        const long long off =
+            data_start_ + (static_cast<long long>(k) * height_ * width_ + static_cast<long long>(fits_row) * width_) *
+                              static_cast<long long>(es);
+        write_all(fd_, line_.data(), line_.size(), off, path_);
+    }
+}
+
+void FitsCubeWriter::write_row(int j, const double* row) { put_row(j, row); }
+void FitsCubeWriter::write_row(int j, const float* row) { put_row(j, row); }
+
+void FitsCubeWriter::finish() {
+    if (fd_ < 0) return;
+    if (::fsync(fd_) != 0 && errno != EINVAL) throw std::runtime_error("flush error on '" + path_ + "'");
+    if (::close(fd_) != 0) throw std::runtime_error("close error on '" + path_ + "'");
+    fd_ = -1;
+}
+
+void write_rgbe(const std::string& path, const float* rgba, int width, int height, float scale) {
+    if (!rgba || width <= 0 || height <= 0) throw std::runtime_error("empty image");
+    std::FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) throw std::runtime_error("cannot open '" + path + "' for writing: " + std::strerror(errno));
+    std::fprintf(f, "#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y %d +X %d\n", height, width);
+    std::vector<unsigned char> px(static_cast<size_t>(width) * 4), out;
+    out.reserve(static_cast<size_t>(width) * 5 + 4);
+    for (int j = 0; j < height; ++j) {
+        for (int i = 0; i < width; ++i) {
+            const float* p = rgba + (static_cast<size_t>(j) * width + i) * 4;
+            float c[3];
+            for (int k = 0; k < 3; ++k) {
+                const float v = p[k] * scale;
+                c[k] = (v > 0.0f && std::isfinite(v)) ? v : 0.0f;   // RGBE holds non-negative finite values
+            }
+            const float m = std::max(c[0], std::max(c[1], c[2]));
+            unsigned char* q = &px[static_cast<size_t>(i) * 4];
+            if (m < 1e-32f) {
+                q[0] = q[1] = q[2] = q[3] = 0;
+            } else {
+                int e;
+                const float s = std::frexp(m, &e) * 256.0f / m;   // Ward's shared-exponent encoding
+                for (int k = 0; k < 3; ++k) q[k] = static_cast<unsigned char>(std::min(255.0f, c[k] * s));
+                q[3] = static_cast<unsigned char>(e + 128);
+            }
+        }
+        out.clear();
+        if (width >= 8 && width < 32768) {
+            // New-style scanline: marker, then each channel as literal runs of <= 128 bytes.
+            out.push_back(2);
+            out.push_back(2);
+            out.push_back(static_cast<unsigned char>(width >> 8));
+            out.push_back(static_cast<unsigned char>(width & 255));
+            for (int ch = 0; ch < 4; ++ch)
+                for (int i = 0; i < width;) {
+                    const int n = std::min(128, width - i);
+                    out.push_back(static_cast<unsigned char>(n));
+                    for (int t = 0; t < n; ++t) out.push_back(px[static_cast<size_t>(i + t) * 4 + ch]);
+                    i += n;
+                }
+        } else {
+            out.assign(px.begin(), px.end());
+        }
+        if (std::fwrite(out.data(), 1, out.size(), f) != out.size()) {
+            std::fclose(f);
+            throw std::runtime_error("write error on '" + path + "'");
+        }
+    }
+    if (std::fclose(f) != 0) throw std::runtime_error("close error on '" + path + "'");
+}
+
+float hdr_exposure_scale(const float* rgba, int width, int height) {
+    double log_sum = 0.0;
+    long long lit = 0;
+    const long long n = static_cast<long long>(width) * height;
+    for (long long i = 0; i < n; ++i) {
+        const double L = 0.2126 * rgba[4 * i] + 0.7152 * rgba[4 * i + 1] + 0.0722 * rgba[4 * i + 2];
+        if (L > 1e-10) {
+            log_sum += std::log(1e-6 + L);
+            ++lit;
+        }
+    }
+    const double L_avg = lit > 0 ? std::exp(log_sum / lit) : 1.0;
+    return static_cast<float>(0.18 / L_avg);
+}
+
+}  // namespace grrt_amd

@@ -1,0 +1,58 @@
+// Output files of the renders: the FITS spectral cube (NOST 100-2.0 primary
+// HDU, the header cards and data order of src/fits_writer.cpp:148-401) and
+// the Radiance RGBE .hdr image (the CLI's linear and exposure-scaled HDR
+// outputs, cli/main.cpp:398-434).
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace grrt_amd {
+
+// Metadata cards of a spectral cube (src/fits_writer.cpp:210-224).
+struct FitsMeta {
+    double spin = 0.0, mass = 1.0, observer_r = 50.0, observer_theta = 1.396, fov = 1.047;
+    int samples_per_pixel = 1;
+};
+
+// The header as 80-character cards, END included (not yet padded to 2880).
+// bitpix -64 (double) or -32 (float).
+std::vector<std::string> fits_header_cards(int width, int height, int num_bins, const std::vector<double>& freqs,
+                                           const FitsMeta& meta, int bitpix);
+
+// Row-streamed cube writer: the header is written and the file extended to
+// its final padded size at open; write_row() stores one image row (renderer
+// row j, 0 = top, layout row[i * num_bins + k]) at its place in every
+// frequency plane, bottom row first as FITS counts rows.  Rows may arrive in
+// any order; unwritten rows read as zero.
+class FitsCubeWriter {
+public:
+    FitsCubeWriter(const std::string& path, int width, int height, int num_bins, const std::vector<double>& freqs,
+                   const FitsMeta& meta, int bitpix);
+    ~FitsCubeWriter();
+    FitsCubeWriter(const FitsCubeWriter&) = delete;
+    FitsCubeWriter& operator=(const FitsCubeWriter&) = delete;
+    void write_row(int j, const double* row);
+    void write_row(int j, const float* row);
+    void finish();
+
+private:
+    template <class T>
+    void put_row(int j, const T* row);
+    int fd_ = -1;
+    int width_, height_, bins_, bitpix_;
+    long long data_start_ = 0;
+    std::string path_;
+    std::vector<unsigned char> line_;
+};
+
+// Radiance RGBE image (new-style run-length scanlines), top row first
+// ("-Y height +X width").  rgba: width*height*4 floats; each RGB is scaled
+// by `scale` before encoding.
+void write_rgbe(const std::string& path, const float* rgba, int width, int height, float scale);
+
+// Exposure scale of the CLI's normalised HDR: 0.18 over the log-average
+// luminance of the lit pixels (cli/main.cpp:410-425).
+float hdr_exposure_scale(const float* rgba, int width, int height);
+
+}  // namespace grrt_amd

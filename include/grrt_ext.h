@@ -202,9 +202,27 @@ GRRT_EXPORT int grrt_ext_surrogate_dataset(GRRTContext* ctx, int validation, flo
  * integrator.  A disk hit is shaded with the exact redshift of its predicted
  * radius.  judge_fraction of the shaded samples is traced as well and
  * compared (stats).  out: RGBA float32 as grrt_render (host memory, or
- * device memory when device_out != 0). */
+ * device memory when device_out != 0; NULL with device_out != 0 keeps the
+ * frame in the context's device memory). */
 GRRT_EXPORT int grrt_ext_render_surrogate(GRRTContext* ctx, float* out, int device_out, double entropy_max,
                                           double lng_tol, double judge_fraction, GRRTSurrogateStats* stats);
+
+/* ---- Output files --------------------------------------------------------
+ * grrt_render_spectral_to_fits_cb (include/grrt/api.h) streams the double
+ * cube (BITPIX -64) in bands of rows; this variant streams the float32 cube
+ * of the context's precision setting (BITPIX -32).  Same header cards,
+ * row order and return codes. */
+GRRT_EXPORT int grrt_ext_render_spectral_to_fits_f32(GRRTContext* ctx, const char* path, int width, int height,
+                                                     grrt_progress_fn progress, void* user_data);
+/* Host-only: writes a cube in renderer layout [(j*width + i)*num_bins + k]
+ * (double for bitpix -64, float for -32) as FITS, metadata cards from meta
+ * (NULL: defaults).  Returns 0, or -1 with grrt_last_error(). */
+GRRT_EXPORT int grrt_ext_write_fits_cube(const char* path, const void* cube, int bitpix, int width, int height,
+                                         int num_bins, const double* frequencies_hz, const GRRTParams* meta);
+/* Host-only: writes an RGBA float frame as a Radiance RGBE .hdr (top row
+ * first), linear (normalised=0) or scaled by 0.18 over the log-average
+ * luminance of the lit pixels (normalised=1), the CLI's two HDR outputs. */
+GRRT_EXPORT int grrt_ext_write_hdr(const char* path, const float* rgba, int width, int height, int normalised);
 
 #ifdef __cplusplus
 }

@@ -135,3 +135,32 @@ def test_c3_spectral_cube_full_size(grrt, oracle):
     print("C3 per-frequency median rel err:", " ".join(f"{e:.1e}" for e in errs),
           f"trace {st.trace_ms:.1f} ms", flush=True)
     assert max(errs) <= 1e-3
+
+
+def test_spectral_fits_stream_equals_cube(grrt, oracle, tmp_path):
+    """grrt_render_spectral_to_fits_cb streams the same cube grrt_render_spectral
+    returns (BITPIX -64, bit for bit), and the float32 variant the mixed cube."""
+    print("case: spectral FITS stream 1024x40 x 8 bins (two bands of rows)", flush=True)
+    freqs = FREQS[::4]
+    W, H = 1024, 40
+    p, _, _ = _setup(grrt, oracle, W, H, freqs=freqs)
+    with grrt.Context(p) as ctx:
+        ctx.set_frequency_bins(freqs)
+        cube = ctx.render_spectral()
+        seen = []
+        path = str(tmp_path / "s.fits")
+        assert ctx.render_spectral_to_fits(path, progress=seen.append) == 0
+        ctx.set_precision(True)
+        cube32 = ctx.render_spectral_f32()
+        path32 = str(tmp_path / "s32.fits")
+        assert ctx.render_spectral_to_fits(path32, f32=True) == 0
+    assert seen and seen[-1] == 1.0
+    for pth, ref, dt in ((path, cube, ">f8"), (path32, cube32, ">f4")):
+        raw = open(pth, "rb").read()
+        off = 0
+        while not raw[off:off + 80].startswith(b"END "):
+            off += 80
+        hdr = (off + 80 + 2879) // 2880 * 2880
+        n = W * H * len(freqs)
+        data = np.frombuffer(raw, dtype=dt, count=n, offset=hdr).reshape(len(freqs), H, W)
+        assert np.array_equal(data[:, ::-1, :].transpose(1, 2, 0), ref)
