@@ -116,7 +116,7 @@ class GRRTTrainStats(C.Structure):
 
 class GRRTSurrogateStats(C.Structure):
     _fields_ = [
-        ("infer_ms", C.c_double), ("trace_ms", C.c_double), ("judge_ms", C.c_double),
+        ("infer_ms", C.c_double), ("trace_ms", C.c_double), ("gpu_ms", C.c_double),
         ("total_ms", C.c_double), ("rays", C.c_int64), ("fallback_rays", C.c_int64),
         ("judged_rays", C.c_int64), ("trace_attempts", C.c_int64),
         ("judge_class_agreement", C.c_double), ("judge_median_rel_err", C.c_double),

@@ -461,7 +461,9 @@ GRRTContext* grrt_create(const GRRTParams* params) {
         check(hipDeviceGetAttribute(&ctx->num_cus, hipDeviceAttributeMultiprocessorCount, ctx->device),
               "hipDeviceGetAttribute");
         check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+        check(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking), "hipStreamCreate");
         for (auto& e : ctx->ev) check(hipEventCreate(&e), "hipEventCreate");
+        for (auto& e : ctx->band_ev) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
         void* flag = nullptr;
         check(hipHostMalloc(&flag, sizeof(int), hipHostMallocMapped | hipHostMallocCoherent), "hipHostMalloc");
         ctx->cancel_flag = static_cast<volatile int*>(flag);
@@ -525,6 +527,9 @@ void grrt_destroy(GRRTContext* ctx) {
     if (ctx->cancel_flag) (void)hipHostFree(const_cast<int*>(ctx->cancel_flag));
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : ctx->band_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

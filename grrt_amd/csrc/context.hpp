@@ -62,7 +62,9 @@ struct GRRTContext {
     int device = 0;
     int num_cus = 0;
     hipStream_t stream = nullptr;
+    hipStream_t stream2 = nullptr;           // second stream (default priority): surrogate fallback traces
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t band_ev[4] = {nullptr, nullptr, nullptr, nullptr};   // cross-stream ordering, no timing
 
     // Scene tables in device memory.
     grrt_amd::CameraFrame cam{};

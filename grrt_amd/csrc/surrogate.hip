@@ -318,6 +318,18 @@ __device__ __forceinline__ void interleave_window() {
     }
 }
 
+// Ray visited at position l of an inference launch: in order, or row by
+// row from the image centre outwards (H/2, H/2-1, H/2+1, ...) so the rays
+// around the shadow, whose fallback traces are the longest, are queued for
+// the integrator first.
+__device__ __forceinline__ long long visit_ray(const InferArgs& IA, long long l) {
+    if (IA.row_rays <= 0) return IA.ray_base + l;
+    const long long k = l / IA.row_rays, col = l - k * IA.row_rays;
+    const long long half = IA.rows / 2;
+    const long long row = (k & 1) ? half - (k + 1) / 2 : half + k / 2;
+    return IA.ray_base + row * IA.row_rays + col;
+}
+
 // One 32-neuron block of a layer from a weight window: the bias, then the
 // 16 k-step MFMA chain with the window's reads issued three k-steps ahead.
 __device__ __forceinline__ f32x16 window_block(const bf16x8* W, const bf16x8 (&in)[16], const float* bias, int mt,
@@ -362,12 +374,19 @@ __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
     if (tid < 16) s_bias[4 * kHid + tid] = tid < kOut ? A.params[oB4 + tid] : 0.0f;
     const long long n = A.n;
     const long long n_tiles = (n + kInferTile - 1) / kInferTile;
-    if (static_cast<long long>(blockIdx.x) < n_tiles) load_window(chunk_src(A, 0), wbuf[0]);
+    // Tiles are claimed dynamically (one claim ahead), so blocks that start
+    // late, while other kernels hold part of the chip, simply take fewer.
+    __shared__ long long s_claim;
+    if (tid == 0) s_claim = static_cast<long long>(atomicAdd(IA.tile_counter, 1ull));
+    __syncthreads();
+    long long tile = s_claim;
+    if (tile < n_tiles) load_window(chunk_src(A, 0), wbuf[0]);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     int buf = 0;
-    for (long long tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
-        const bool more_tiles = tile + gridDim.x < n_tiles;
+    while (tile < n_tiles) {
+        if (tid == 0) s_claim = static_cast<long long>(atomicAdd(IA.tile_counter, 1ull));
+        bool more_tiles = true;         // read from s_claim after the first window's barrier
         const long long ray0 = tile * kInferTile + wave * 32;
         float x[1][kIn];
         {
@@ -377,7 +396,7 @@ __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
             long long i = ray0 + c;
             if (i >= n) i = n - 1;
             SampleId d;
-            decode_ray(V, i, d);
+            decode_ray(V, visit_ray(IA, i), d);
             double px, py;
             sample_position(V, d, px, py);
             const double alpha = (px - V.W / 2.0) * V.fov / V.W;
@@ -410,6 +429,7 @@ __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
         // the other buffer, run this window's block (and the SiLU of the
         // previous block), then wait for the copy and publish it.
         auto window = [&](int cidx, auto&& body) {
+            if (cidx == kChunks - 1) more_tiles = s_claim < n_tiles;
             const bool has_next = cidx + 1 < kChunks || more_tiles;
             if (has_next) load_window(chunk_src(A, cidx + 1 < kChunks ? cidx + 1 : 0), wbuf[buf ^ 1]);
             body(reinterpret_cast<const bf16x8*>(wbuf[buf]));
@@ -474,8 +494,8 @@ __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
         __builtin_amdgcn_wave_barrier();
         asm volatile("" ::: "memory");
         // One lane per ray: lanes 0..31 of the wave.
-        const long long i = ray0 + lane;
-        const bool valid = lane < 32 && i < n;
+        const bool valid = lane < 32 && ray0 + lane < n;
+        const long long i = visit_ray(IA, valid ? ray0 + lane : 0);
         float y[kOut];
 #pragma unroll
         for (int m = 0; m < kOut; ++m) y[m] = outs[wave][lane & 31][m];
@@ -529,6 +549,9 @@ __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
             P.sample_rgb[3 * i + 2] = col[2];
             P.sample_cls[i] = static_cast<uint8_t>(cls);
         }
+        const long long next = s_claim;
+        __syncthreads();                // everyone has read the claim before it is replaced
+        tile = next;
     }
 }
 
@@ -850,6 +873,7 @@ hipError_t launch_mlp_forward(const MlpArgs& a, bool train, hipStream_t st) {
 }
 
 hipError_t launch_mlp_infer(const InferArgs& a, int num_cus, hipStream_t st) {
+    if (a.mlp.n <= 0) return hipSuccess;
     const long long tiles = (a.mlp.n + kInferTile - 1) / kInferTile;
     const long long grid = std::min<long long>(tiles, 2LL * num_cus);
     hipLaunchKernelGGL(mlp_infer_kernel, dim3(static_cast<unsigned>(std::max<long long>(grid, 1))), dim3(256), 0, st,

@@ -105,10 +105,14 @@ struct InferArgs {
     double lng_tol;                     // |ln g_net - ln g(r_net)| above it falls back (<= 0: off)
     uint32_t judge_threshold;           // hash < threshold (of 2^32) selects a judged ray
     uint32_t frame_seed;
+    long long ray_base;                 // first ray of the launch (rays ray_base .. ray_base + mlp.n)
+    long long row_rays;                 // > 0: visit whole rows centre-out (rays per row); 0: in order
+    int rows;                           // image rows (centre-out order)
     long long* fb_list;                 // fallback ray ids
     long long* judge_list;              // judged ray ids (shaded by the surrogate)
     unsigned long long* counts;         // [0] fallbacks, [1] judged
     long long judge_cap;
+    unsigned long long* tile_counter;   // dynamic 128-ray tile claims (zeroed before the launch)
 };
 
 hipError_t launch_pack(const float* params, __bf16* pack0, __bf16* packH, __bf16* pack4, __bf16* WT,

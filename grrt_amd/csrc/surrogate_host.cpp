@@ -447,16 +447,27 @@ int grrt_ext_render_surrogate(GRRTContext* ctx, float* out, int device_out, doub
         const long long npix = static_cast<long long>(r.W) * r.H;
         const long long n_rays = npix * nS;
         judge_fraction = std::clamp(judge_fraction, 0.0, 1.0);
-        const long long cap = static_cast<long long>(judge_fraction * 1.25 * n_rays) + 4096;
+        // One band of all rays, visited row by row from the image centre
+        // outwards: the rays around the shadow and the inner disk, whose
+        // fallback traces are the longest, lead the fallback list and start
+        // first in the integrator launch that follows the network pass.
+        struct Band {
+            long long r0, r1, cap, joff;
+        };
+        std::vector<Band> bands;
+        const long long row_rays = static_cast<long long>(r.W) * nS;
+        bands.push_back(Band{0, n_rays, static_cast<long long>(judge_fraction * 1.25 * static_cast<double>(n_rays)) + 4096, 0});
+        const int nb = static_cast<int>(bands.size());
+        const long long cap = bands.back().joff + bands.back().cap;
         check(ctx->sample_rgb.ensure(static_cast<size_t>(3 * (n_rays + cap))), "hipMalloc(samples)");
         check(ctx->sample_cls.ensure(static_cast<size_t>(n_rays + cap)), "hipMalloc(classes)");
         check(ctx->frame.ensure(static_cast<size_t>(4 * npix)), "hipMalloc(frame)");
-        check(ctx->counters.ensure(1), "hipMalloc(counters)");
-        check(ctx->params_dev.ensure(1), "hipMalloc(params)");
+        check(ctx->counters.ensure(static_cast<size_t>(nb)), "hipMalloc(counters)");
+        check(ctx->params_dev.ensure(static_cast<size_t>(nb)), "hipMalloc(params)");
         check(S.fb_list.ensure(static_cast<size_t>(n_rays)), "hipMalloc(fallback list)");
         check(S.judge_list.ensure(static_cast<size_t>(cap)), "hipMalloc(judge list)");
         check(S.judge_err.ensure(static_cast<size_t>(cap)), "hipMalloc(judge errors)");
-        check(S.counts.ensure(4), "hipMalloc(counts)");
+        check(S.counts.ensure(static_cast<size_t>(4 * nb)), "hipMalloc(counts)");
 
         TraceParams P{};
         grrt_api::fill_view(ctx, P.view, r.a, r.r_obs, r.th_obs, r.ph_obs, r.fov, r.W, r.H, ctx->cam, ctx->flux.ptr,
@@ -473,57 +484,88 @@ int grrt_ext_render_surrogate(GRRTContext* ctx, float* out, int device_out, doub
         P.sample_rgb = ctx->sample_rgb.ptr;
         P.sample_cls = ctx->sample_cls.ptr;
         P.records = nullptr;
-        P.counters = ctx->counters.ptr;
         P.cancel = nullptr;
-        P.fb_list = S.fb_list.ptr;
-        P.judge_list = S.judge_list.ptr;
-        P.list_counts = S.counts.ptr;
-        P.judge_cap = cap;
         P.refill_min = std::clamp(ctx->tune_refill, 1, 64);
         P.chunk = std::max(ctx->tune_chunk, 64);
-        check(hipMemcpy(ctx->params_dev.ptr, &P, sizeof(TraceParams), hipMemcpyHostToDevice), "upload params");
+        std::vector<TraceParams> bp(nb, P);
+        for (int k = 0; k < nb; ++k) {
+            bp[k].counters = ctx->counters.ptr + k;
+            bp[k].fb_list = S.fb_list.ptr + bands[k].r0;
+            bp[k].judge_list = S.judge_list.ptr + bands[k].joff;
+            bp[k].list_counts = S.counts.ptr + 4 * k;
+            bp[k].judge_cap = bands[k].cap;
+            bp[k].judge_out = n_rays + bands[k].joff;
+        }
+        check(hipMemcpy(ctx->params_dev.ptr, bp.data(), sizeof(TraceParams) * nb, hipMemcpyHostToDevice),
+              "upload params");
 
-        InferArgs IA{};
-        IA.mlp = net_args(S);
-        IA.mlp.n = n_rays;
-        IA.P = ctx->params_dev.ptr;
-        IA.entropy_max = entropy_max;
-        IA.lng_tol = lng_tol;
-        IA.judge_threshold = static_cast<uint32_t>(std::min(judge_fraction * 4294967296.0, 4294967295.0));
-        IA.frame_seed = ++S.frame_no;
-        IA.fb_list = S.fb_list.ptr;
-        IA.judge_list = S.judge_list.ptr;
-        IA.counts = S.counts.ptr;
-        IA.judge_cap = cap;
-
+        const uint32_t thr = static_cast<uint32_t>(std::min(judge_fraction * 4294967296.0, 4294967295.0));
+        const uint32_t frame = ++S.frame_no;
+        const hipStream_t s2 = ctx->stream2;
         check(launch_reset(ctx->counters.ptr, ctx->sample_cls.ptr, n_rays + cap, st), "reset");
-        check(launch_reset_counts(S.counts.ptr, 4, st), "reset counts");
+        for (int k = 1; k < nb; ++k) check(launch_reset(ctx->counters.ptr + k, nullptr, 0, st), "reset");
+        check(launch_reset_counts(S.counts.ptr, 4 * nb, st), "reset counts");
         check(hipEventRecord(ctx->ev[0], st), "event");
-        check(launch_mlp_infer(IA, ctx->num_cus, st), "surrogate pass");
-        check(hipEventRecord(ctx->ev[1], st), "event");
         const int blocks = std::max(1, ctx->num_cus * ctx->tune_blocks_per_cu);
-        check(launch_trace_thin(ctx->params_dev.ptr, 2, blocks, ctx->tune_block, st), "fallback trace");
-        check(hipEventRecord(ctx->ev[2], st), "event");
-        check(launch_judge(S.judge_list.ptr, S.counts.ptr, cap, ctx->sample_rgb.ptr, ctx->sample_cls.ptr, n_rays,
-                           S.judge_err.ptr, S.counts.ptr + 2, st),
-              "judge");
-        check(launch_finalize(ctx->sample_rgb.ptr, ctx->sample_cls.ptr, ctx->frame.ptr, npix, nS, st), "finalize");
+        for (int k = 0; k < nb; ++k) {
+            InferArgs IA{};
+            IA.mlp = net_args(S);
+            IA.mlp.n = bands[k].r1 - bands[k].r0;
+            IA.P = ctx->params_dev.ptr + k;
+            IA.entropy_max = entropy_max;
+            IA.lng_tol = lng_tol;
+            IA.judge_threshold = thr;
+            IA.frame_seed = frame;
+            IA.ray_base = bands[k].r0;
+            IA.row_rays = row_rays;
+            IA.rows = r.H;
+            IA.fb_list = S.fb_list.ptr + bands[k].r0;
+            IA.judge_list = S.judge_list.ptr + bands[k].joff;
+            IA.counts = S.counts.ptr + 4 * k;
+            IA.judge_cap = bands[k].cap;
+            IA.tile_counter = S.counts.ptr + 4 * k + 3;
+            check(launch_mlp_infer(IA, ctx->num_cus, st), "surrogate pass");
+            check(hipEventRecord(ctx->band_ev[k], st), "event");
+            check(hipStreamWaitEvent(s2, ctx->band_ev[k], 0), "stream wait");
+            check(launch_trace_thin(ctx->params_dev.ptr + k, 2, blocks, ctx->tune_block, s2), "fallback trace");
+            check(launch_judge(S.judge_list.ptr + bands[k].joff, S.counts.ptr + 4 * k, bands[k].cap,
+                               ctx->sample_rgb.ptr, ctx->sample_cls.ptr, n_rays + bands[k].joff,
+                               S.judge_err.ptr + bands[k].joff, S.counts.ptr + 4 * k + 2, s2),
+                  "judge");
+        }
+        check(hipEventRecord(ctx->ev[1], st), "event");
+        check(launch_finalize(ctx->sample_rgb.ptr, ctx->sample_cls.ptr, ctx->frame.ptr, npix, nS, s2), "finalize");
+        check(hipEventRecord(ctx->ev[2], s2), "event");
         check(hipEventSynchronize(ctx->ev[2]), "surrogate frame");
-        check(hipStreamSynchronize(st), "surrogate frame");
-        unsigned long long cnt[4];
-        check(hipMemcpy(cnt, S.counts.ptr, sizeof(cnt), hipMemcpyDeviceToHost), "read counts");
-        DevCounters c{};
-        check(hipMemcpy(&c, ctx->counters.ptr, sizeof(c), hipMemcpyDeviceToHost), "read counters");
-        const long long nj = static_cast<long long>(std::min<unsigned long long>(cnt[1], static_cast<unsigned long long>(cap)));
-        std::vector<float> errs(static_cast<size_t>(nj));
-        if (nj) check(hipMemcpy(errs.data(), S.judge_err.ptr, sizeof(float) * nj, hipMemcpyDeviceToHost), "judge");
+        std::vector<unsigned long long> cnt(static_cast<size_t>(4 * nb));
+        check(hipMemcpy(cnt.data(), S.counts.ptr, sizeof(unsigned long long) * cnt.size(), hipMemcpyDeviceToHost),
+              "read counts");
+        std::vector<DevCounters> cs(static_cast<size_t>(nb));
+        check(hipMemcpy(cs.data(), ctx->counters.ptr, sizeof(DevCounters) * nb, hipMemcpyDeviceToHost),
+              "read counters");
+        unsigned long long n_fb = 0, nj = 0, n_agree = 0, attempts = 0;
+        std::vector<float> errs;
+        for (int k = 0; k < nb; ++k) {
+            const unsigned long long jk =
+                std::min<unsigned long long>(cnt[4 * k + 1], static_cast<unsigned long long>(bands[k].cap));
+            n_fb += cnt[4 * k];
+            nj += jk;
+            n_agree += cnt[4 * k + 2];
+            attempts += cs[k].attempts;
+            const size_t o = errs.size();
+            errs.resize(o + jk);
+            if (jk)
+                check(hipMemcpy(errs.data() + o, S.judge_err.ptr + bands[k].joff, sizeof(float) * jk,
+                                hipMemcpyDeviceToHost),
+                      "judge");
+        }
         if (out)
             check(hipMemcpy(out, ctx->frame.ptr, sizeof(float) * 4 * npix,
                             device_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost),
                   "copy frame");
         float ms[2] = {0.0f, 0.0f};
         check(hipEventElapsedTime(&ms[0], ctx->ev[0], ctx->ev[1]), "event time");
-        check(hipEventElapsedTime(&ms[1], ctx->ev[1], ctx->ev[2]), "event time");
+        check(hipEventElapsedTime(&ms[1], ctx->ev[0], ctx->ev[2]), "event time");
         std::vector<float> lit;
         lit.reserve(errs.size());
         for (float e : errs)
@@ -537,12 +579,13 @@ int grrt_ext_render_surrogate(GRRTContext* ctx, float* out, int device_out, doub
             GRRTSurrogateStats& s = *stats;
             s = GRRTSurrogateStats{};
             s.infer_ms = ms[0];
-            s.trace_ms = ms[1];
+            s.trace_ms = std::max(0.0f, ms[1] - ms[0]);
+            s.gpu_ms = ms[1];
             s.rays = n_rays;
-            s.fallback_rays = static_cast<int64_t>(cnt[0]);
-            s.judged_rays = nj;
-            s.trace_attempts = static_cast<int64_t>(c.attempts);
-            s.judge_class_agreement = nj ? static_cast<double>(cnt[2]) / nj : 1.0;
+            s.fallback_rays = static_cast<int64_t>(n_fb);
+            s.judged_rays = static_cast<int64_t>(nj);
+            s.trace_attempts = static_cast<int64_t>(attempts);
+            s.judge_class_agreement = nj ? static_cast<double>(n_agree) / nj : 1.0;
             s.judge_median_rel_err = med;
             s.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         }

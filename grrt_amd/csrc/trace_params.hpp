@@ -116,6 +116,7 @@ struct TraceParams {
     const long long* judge_list;
     const unsigned long long* list_counts;   // [0] fallbacks, [1] judged
     long long judge_cap;
+    long long judge_out;                // sample index of the first judged result
 
     int refill_min;                     // idle lanes that trigger a refill
     int chunk;                          // rays claimed per queue atomic (>= 64)

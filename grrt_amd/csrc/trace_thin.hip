@@ -282,8 +282,7 @@ void trace_thin(const TraceParams* Pg) {
                             r_term = V.r_term;
                         }
                         oid = V.ray_offset + d.local;
-                        if (kList && mine >= n_fb)
-                            oid = V.n_rays + static_cast<long long>(mine - n_fb);
+                        if (kList && mine >= n_fb) oid = Pg->judge_out + static_cast<long long>(mine - n_fb);
                         init_ray(V, d, K, R);
                         live = true;
                         const int term = termination(max_steps, r_term, r_escape, R);

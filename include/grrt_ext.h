@@ -164,7 +164,10 @@ typedef struct {
 } GRRTTrainStats;
 
 typedef struct {
-    double infer_ms, trace_ms, judge_ms, total_ms;
+    double infer_ms;          /* GPU time from the frame's start until the network pass ends */
+    double trace_ms;          /* FP64 fallback/judge tracing still running after that */
+    double gpu_ms;            /* GPU time of the whole frame (infer_ms + trace_ms) */
+    double total_ms;          /* wall time of the call */
     int64_t rays, fallback_rays, judged_rays, trace_attempts;
     double judge_class_agreement;     /* judged rays: surrogate class == FP64 class */
     double judge_median_rel_err;      /* judged lit rays: |L_s - L| / L of summed RGB */
