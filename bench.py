@@ -35,6 +35,10 @@ BF16_PEAK_TFLOPS = 2500.0         # MI355X dense BF16 MFMA peak
 SURROGATE_FLOPS_PER_RAY = 2.0 * (5 * 256 + 3 * 256 * 256 + 256 * 9)
 # Surrogate gates: class entropy (nats), redshift consistency (ln g), judged share.
 SG_ENTROPY, SG_LNG_TOL, SG_JUDGE = 0.1, 0.005, 0.02
+# HBM bytes moved by one plain-frame trace launch: rocprofv3 --pmc FETCH_SIZE
+# (3 816 KiB, x2 by a copy-kernel calibration) + WRITE_SIZE (103 689 KiB,
+# x1), i.e. the 25-byte per-sample outputs plus table refills.
+TRACE_HBM_BYTES = (3816 * 2 + 103689) * 1024
 
 
 def c2_params(grrt):
@@ -259,7 +263,7 @@ def main() -> int:
         "host_copy_inclusive_ms": round(host_ms, 3),
         "roofline": {"bound": "valu", "achieved": round(achieved, 3),
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None},
+                     "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": TRACE_HBM_BYTES},
     }
     if not args.no_surrogate:
         out["surrogate"] = surrogate_section(grrt, ctx, value, args.steps, args.warmup, args.sg_steps,

@@ -737,6 +737,23 @@ __global__ void reduce_splits_kernel(const float* __restrict__ part, int splits,
     out[i] = s;
 }
 
+// Few outputs, many partials: one block per output, threads over the
+// partials, then a tree in LDS.
+__global__ __launch_bounds__(256) void reduce_splits_wide_kernel(const float* __restrict__ part, int splits,
+                                                                 long long count, float* __restrict__ out) {
+    __shared__ float red[256];
+    const long long i = blockIdx.x;
+    float s = 0.0f;
+    for (int z = threadIdx.x; z < splits; z += 256) s += part[static_cast<long long>(z) * count + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int stp = 128; stp > 0; stp >>= 1) {
+        if (static_cast<int>(threadIdx.x) < stp) red[threadIdx.x] += red[threadIdx.x + stp];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[i] = red[0];
+}
+
 // Layer-0 gradients in FP32 for a block of 64 rays: part[block][m*5+i] =
 // sum_b dz0[b][m] x[b][i] and part[block][1280+m] = sum_b dz0[b][m] (the
 // parameter order of W0 then b0).  dH0T, Z0T neuron-major.
@@ -910,6 +927,11 @@ hipError_t launch_gemm_nt(int M, int N, int K, const __bf16* A, long long lda, c
 }
 
 hipError_t launch_reduce_splits(const float* part, int splits, long long count, float* out, hipStream_t st) {
+    if (count <= 4096 && splits >= 64) {
+        hipLaunchKernelGGL(reduce_splits_wide_kernel, dim3(static_cast<unsigned>(count)), dim3(256), 0, st, part,
+                           splits, count, out);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(reduce_splits_kernel, dim3(blocks_for(count, 256)), dim3(256), 0, st, part, splits, count, out);
     return hipGetLastError();
 }
