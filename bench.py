@@ -193,6 +193,68 @@ def surrogate_section(grrt, ctx, plain_value, steps, warmup, sg_steps, c5_frames
     return out
 
 
+def c4_views(seed: int = 7, n: int = 64):
+    """The batched-views workload: n thin-disk Kerr views drawn from one
+    PRNG stream in a fixed order per view (a, theta, phi, log10 r, fov,
+    resolution): a ~ U[0, 0.998], theta ~ U[5, 89] deg, phi ~ U[0, 360) deg,
+    r log-uniform in [20, 100], fov ~ U[30, 90] deg, square side from
+    {128, 256, 512, 1024}."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        a = rng.uniform(0.0, 0.998)
+        th = math.radians(rng.uniform(5.0, 89.0))
+        ph = math.radians(rng.uniform(0.0, 360.0))
+        r = 10.0 ** rng.uniform(math.log10(20.0), math.log10(100.0))
+        fov = math.radians(rng.uniform(30.0, 90.0))
+        side = int(rng.choice([128, 256, 512, 1024]))
+        out.append((a, r, th, ph, fov, side, side))
+    return out
+
+
+def c4_section(grrt, reps: int) -> dict:
+    """64 uneven views in one persistent launch (recorded graph, autotuned
+    launch shape) against the same views rendered one after another."""
+    views = c4_views()
+    rays = sum(v[5] * v[6] for v in views)
+    base = grrt.make_params(width=64, height=64, metric=grrt.METRIC_KERR, spin=0.5, disk=True, disk_outer=20.0,
+                            stars=False, tolerance=1e-8, max_steps=10000)
+    with grrt.Context(base) as ctx:
+        ctx.set_sampling(grrt.SAMPLING_CENTER)
+        best, tune_ms = ctx.autotune(views)
+        ctx.render_batch(views, device=True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.render_batch(views, device=True)
+        batch_ms = (time.perf_counter() - t0) / reps * 1e3
+        kernel_ms = ctx.stats().trace_ms
+    # The same views one at a time: a context per view (created outside the
+    # timed loop), each frame left in device memory like the batch's.
+    ctxs = []
+    for a, r, th, ph, fov, w, h in views:
+        p = grrt.make_params(width=w, height=h, metric=grrt.METRIC_KERR, spin=a, observer_r=r, observer_theta=th,
+                             observer_phi=ph, fov=fov, disk=True, disk_outer=20.0, stars=False, tolerance=1e-8,
+                             max_steps=10000)
+        c = grrt.Context(p)
+        c.set_sampling(grrt.SAMPLING_CENTER)
+        ctxs.append(c)
+    for c in ctxs[:2]:
+        c.render_device()
+    t0 = time.perf_counter()
+    for c in ctxs:
+        c.render_device()
+    seq_ms = (time.perf_counter() - t0) * 1e3
+    for c in ctxs:
+        c.close()
+    return {"workload": "64 thin-disk Kerr views from seed 7 (a, theta, phi, r, fov, side from "
+                        "{128..1024}), pixel centres, tol 1e-8",
+            "rays": rays, "batch_mrays_s": round(rays / batch_ms / 1e3, 3), "batch_ms": round(batch_ms, 3),
+            "batch_kernel_ms": round(kernel_ms, 3), "autotuned": best, "autotune_best_ms": round(tune_ms, 3),
+            "sequential_mrays_s": round(rays / seq_ms / 1e3, 3), "sequential_ms": round(seq_ms, 3),
+            "batch_speedup": round(seq_ms / batch_ms, 3)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -203,6 +265,7 @@ def main() -> int:
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--sg-steps", type=int, default=60000, help="surrogate training steps")
     ap.add_argument("--c5-frames", type=int, default=2, help="2048^2 frames timed per form (0: skip)")
+    ap.add_argument("--c4-reps", type=int, default=2, help="replays of the 64-view batch (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
@@ -268,9 +331,11 @@ def main() -> int:
     if not args.no_surrogate:
         out["surrogate"] = surrogate_section(grrt, ctx, value, args.steps, args.warmup, args.sg_steps,
                                              args.c5_frames)
+    ctx.close()
+    if args.c4_reps > 0:
+        out["batched_views"] = c4_section(grrt, args.c4_reps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
-    ctx.close()
     print(json.dumps(out), flush=True)
     return 0
 

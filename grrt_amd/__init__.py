@@ -333,10 +333,14 @@ class Context:
         self._check(self._L.grrt_ext_set_tuning(self._h, block, blocks_per_cu, refill_min, chunk, tile),
                     "set_tuning")
 
-    def render_batch(self, views) -> list[np.ndarray]:
+    def render_batch(self, views, device: bool = False) -> list[np.ndarray] | None:
         """Render several views in one persistent launch (grrt_ext_render_batch);
-        returns one (height, width, 4) float32 frame per view."""
+        returns one (height, width, 4) float32 frame per view, or None with
+        device=True (frames stay in the context's device memory)."""
         arr = views_array(views)
+        if device:
+            self._check(self._L.grrt_ext_render_batch(self._h, arr, len(arr), None, 1), "grrt_ext_render_batch")
+            return None
         npx = sum(v.width * v.height for v in arr)
         out = np.zeros(4 * npx, np.float32)
         self._check(self._L.grrt_ext_render_batch(self._h, arr, len(arr), out.ctypes.data, 0),

@@ -968,12 +968,15 @@ extern "C" {
 
 int grrt_ext_render_batch(GRRTContext* ctx, const GRRTView* views, int n, float* out, int device_out) {
     return guarded(ctx, [&] {
-        if (!views || n <= 0 || !out) throw HipFailure("grrt_ext_render_batch: empty batch or null output");
+        if (!views || n <= 0 || (!out && !device_out))
+            throw HipFailure("grrt_ext_render_batch: empty batch or null output");
         if (!same_batch(ctx, views, n)) prepare_batch(ctx, views, n);
         run_batch(ctx);
         const size_t bytes = static_cast<size_t>(4 * ctx->batch_pixels) * sizeof(float);
-        check(hipMemcpy(out, ctx->batch_out.ptr, bytes, device_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost),
-              "copy batch frames");
+        if (out)
+            check(hipMemcpy(out, ctx->batch_out.ptr, bytes,
+                            device_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost),
+                  "copy batch frames");
         return 0;
     });
 }

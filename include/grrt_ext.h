@@ -96,7 +96,8 @@ typedef struct {
  * launch sequence (parameter upload, reset, trace, finalize) is recorded as
  * a HIP graph and replayed while the views stay the same.  out: the RGBA
  * float32 frames in view order, sum(width*height*4) floats; host memory,
- * or device memory when device_out != 0. */
+ * or device memory when device_out != 0 (NULL with device_out != 0 keeps
+ * the frames in the context's device memory). */
 GRRT_EXPORT int grrt_ext_render_batch(GRRTContext* ctx, const GRRTView* views, int n, float* out,
                                       int device_out);
 
