@@ -504,8 +504,10 @@ class Context:
         self._freqs = arr
         self._L.grrt_set_frequency_bins(self._h, arr.ctypes.data_as(C.POINTER(C.c_double)), len(arr))
 
-    def set_precision(self, mixed: bool):
-        self._check(self._L.grrt_ext_set_precision(self._h, 1 if mixed else 0), "set_precision")
+    def set_precision(self, mode):
+        """0 / False: all FP64; 1 / True: FP32 transfer; 2: FP32 transfer and
+        FP32 far-field geodesic steps (volumetric path)."""
+        self._check(self._L.grrt_ext_set_precision(self._h, int(mode)), "set_precision")
 
     def render_spectral_f32(self, width=None, height=None) -> np.ndarray:
         w = self.params.width if width is None else width

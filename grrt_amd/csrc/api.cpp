@@ -242,6 +242,7 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
     if (vol) {
         P.vol = ctx->vol;
         P.nbins = nb;
+        P.far_f32 = ctx->far_field_f32 ? 1 : 0;
         if (spectral) {
             std::vector<double> lg(nb);
             for (int k = 0; k < nb; ++k) lg[k] = std::log10(ctx->frequency_bins[k]);
@@ -734,8 +735,9 @@ int grrt_ext_set_tuning(GRRTContext* ctx, int block, int blocks_per_cu, int refi
 
 int grrt_ext_set_precision(GRRTContext* ctx, int mode) {
     return guarded(ctx, [&] {
-        if (mode != 0 && mode != 1) throw HipFailure("precision mode must be 0 (FP64) or 1 (mixed)");
-        ctx->mixed_precision = mode == 1;
+        if (mode < 0 || mode > 2) throw HipFailure("precision mode must be 0 (FP64), 1 or 2 (mixed)");
+        ctx->mixed_precision = mode >= 1;
+        ctx->far_field_f32 = mode == 2;
         return 0;
     });
 }

@@ -82,6 +82,7 @@ struct GRRTContext {
     grrt_amd::DevBuf<double> sample_spec;
     grrt_amd::DevBuf<unsigned char> spec_out;
     bool mixed_precision = false;        // FP32 radiative transfer in the volumetric path
+    bool far_field_f32 = false;          // and FP32 geodesic steps outside r = 15 M
 
     // Per-render buffers.
     grrt_amd::DevBuf<double> sample_rgb;

@@ -124,6 +124,7 @@ struct TraceParams {
     // Volumetric disk (the volumetric tracer only).
     VolDev vol;
     int nbins;                          // 3 for RGB, else spectral bins
+    int far_f32;                        // FP32 transfer path: FP32 geodesic steps outside r = 15 M
     const double* nu_obs;               // observer-frame frequencies [Hz]
     const double* lognu_obs;            // log10 of the same
     void* rt_scratch;                   // spectral accumulators, 4 * nbins * n_rays of RT

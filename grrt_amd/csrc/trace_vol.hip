@@ -10,7 +10,11 @@
 // thin-disk kernel; classic RK4 inside the ray-march).  Radiative transfer
 // runs in the template type RT: double for the all-FP64 path, float for the
 // mixed-precision path, where each bin's intensity is accumulated with
-// Neumaier compensated summation.
+// Neumaier compensated summation.  On request (far_f32) the mixed path also
+// takes the outer Dormand–Prince steps in FP32 while the ray is outside
+// r = 15 M (far-field segments, before it reaches the disk or after it
+// leaves), with the error tolerance raised to 1e-6, the smallest the FP32
+// error estimate resolves.
 #include <hip/hip_runtime.h>
 
 #include "device_math.hpp"
@@ -101,6 +105,111 @@ __device__ double adaptive_dp45(const KerrK& K, GState& s, double dl, double tol
         }
     }
     return dl;   // not reached
+}
+
+// ---- FP32 far-field step ----
+constexpr double kFarField = 15.0;       // in units of M
+constexpr float kFarTol = 1e-6f;
+
+struct Vec5f {
+    float v[5];
+};
+
+// Only the FP32-transfer instantiation carries the FP32 far-field stepper.
+template <class RT>
+constexpr bool kFarFieldF32 = sizeof(RT) == sizeof(float);
+
+// The Kerr right-hand side of kerr_rhs.hpp in FP32 (same regrouped
+// expressions; library sincos).
+__device__ __forceinline__ Vec5f kerr_rhs_f32(float M, float a, float a2, float twoM, float P0, float P3,
+                                              const Vec5f& y) {
+    const float r = y.v[0], th = y.v[1], pr = y.v[3], pth = y.v[4];
+    float s, c;
+    sincosf(th, &s, &c);
+    s = copysignf(fmaxf(fabsf(s), 1e-10f), s);
+    const float s2 = s * s, c2 = c * c, sc = s * c, r2 = r * r;
+    const float Sig = fmaf(a2, c2, r2);
+    const float Del = fmaf(r, r - twoM, a2);
+    const float R = 1.0f / (Sig * Del * s2);
+    const float iS = (Del * s2) * R;
+    const float iS2 = iS * iS;
+    const float twoMr = twoM * r, twoMar = twoMr * a, as2 = a * s2;
+    const float nt = fmaf(twoMar, fmaf(as2, P0, P3), ((r2 + a2) * Sig) * P0);
+    const float dt = -(nt * s2) * R;
+    const float dph = fmaf(-(twoMar * s2), P0, (Sig - twoMr) * P3) * R;
+    Vec5f d;
+    d.v[0] = (Del * iS) * pr;
+    d.v[1] = iS * pth;
+    d.v[2] = dph;
+    const float u = fmaf(-as2, dph, dt);
+    const float v = fmaf(a, dt, -(fmaf(a2, s2, Sig) * dph));
+    const float pr2 = pr * pr, pth2 = pth * pth, dph2 = dph * dph;
+    const float cu = M * fmaf(-2.0f, r2, Sig);
+    const float cr = fmaf(r, Del, -(Sig * (r - M)));
+    d.v[3] = fmaf(fmaf(cu, u * u, fmaf(cr, pr2, r * pth2)), iS2, (r * s2) * dph2);
+    d.v[4] = sc * fmaf(fmaf(twoMr, v * v, -a2 * fmaf(Del, pr2, pth2)), iS2, Del * dph2);
+    return d;
+}
+
+// adaptive_dp45 in FP32: the same tableau, error norm and controller, the
+// state converted from and back to FP64 around the step.
+__device__ double adaptive_dp45_f32(const KerrK& K, GState& s, double dl, int& attempts) {
+    const float M = static_cast<float>(K.M), a = static_cast<float>(K.a), a2 = static_cast<float>(K.a2),
+                twoM = static_cast<float>(K.twoM);
+    const float P0 = static_cast<float>(s.P0), P3 = static_cast<float>(s.P3);
+    Vec5f y;
+    for (int i = 0; i < 5; ++i) y.v[i] = static_cast<float>(s.y.v[i]);
+    const Vec5f k1 = kerr_rhs_f32(M, a, a2, twoM, P0, P3, y);
+    float h = static_cast<float>(dl);
+    for (int retry = 0; retry <= 20; ++retry) {
+        Vec5f st, k2, k3, k4, k5, k6, k7, y5, e;
+        for (int i = 0; i < 5; ++i) st.v[i] = fmaf(k1.v[i], h * float(A21), y.v[i]);
+        k2 = kerr_rhs_f32(M, a, a2, twoM, P0, P3, st);
+        for (int i = 0; i < 5; ++i) st.v[i] = fmaf(k2.v[i], h * float(A32), fmaf(k1.v[i], h * float(A31), y.v[i]));
+        k3 = kerr_rhs_f32(M, a, a2, twoM, P0, P3, st);
+        for (int i = 0; i < 5; ++i)
+            st.v[i] = fmaf(k3.v[i], h * float(A43), fmaf(k2.v[i], h * float(A42), fmaf(k1.v[i], h * float(A41), y.v[i])));
+        k4 = kerr_rhs_f32(M, a, a2, twoM, P0, P3, st);
+        for (int i = 0; i < 5; ++i)
+            st.v[i] = fmaf(k4.v[i], h * float(A54), fmaf(k3.v[i], h * float(A53),
+                          fmaf(k2.v[i], h * float(A52), fmaf(k1.v[i], h * float(A51), y.v[i]))));
+        k5 = kerr_rhs_f32(M, a, a2, twoM, P0, P3, st);
+        for (int i = 0; i < 5; ++i)
+            st.v[i] = fmaf(k5.v[i], h * float(A65), fmaf(k4.v[i], h * float(A64),
+                          fmaf(k3.v[i], h * float(A63), fmaf(k2.v[i], h * float(A62),
+                              fmaf(k1.v[i], h * float(A61), y.v[i])))));
+        k6 = kerr_rhs_f32(M, a, a2, twoM, P0, P3, st);
+        for (int i = 0; i < 5; ++i) {
+            y5.v[i] = fmaf(k6.v[i], h * float(B6), fmaf(k5.v[i], h * float(B5),
+                          fmaf(k4.v[i], h * float(B4), fmaf(k3.v[i], h * float(B3),
+                              fmaf(k1.v[i], h * float(B1), y.v[i])))));
+            e.v[i] = fmaf(k6.v[i], h * float(E6), fmaf(k5.v[i], h * float(E5),
+                         fmaf(k4.v[i], h * float(E4), fmaf(k3.v[i], h * float(E3), k1.v[i] * (h * float(E1))))));
+        }
+        k7 = kerr_rhs_f32(M, a, a2, twoM, P0, P3, y5);
+        float err = 0.0f;
+        for (int i = 0; i < 5; ++i) err = fmaxf(err, fabsf(fmaf(k7.v[i], h * float(E7), e.v[i])) / (fabsf(y5.v[i]) + 1e-10f));
+        ++attempts;
+        bool done = retry == 20;
+        double next = h;
+        if (!done && err <= kFarTol) {
+            float f = (err > 1e-30f) ? static_cast<float>(step_factor_v(kFarTol / err)) : 5.0f;
+            f = fminf(fmaxf(f, 1.0f), 5.0f);
+            next = smin(static_cast<double>(h) * f, 5.0 * smax(fabs(static_cast<double>(y5.v[0])), 1.0));
+            done = true;
+        } else if (!done) {
+            h = h * fminf(fmaxf(static_cast<float>(step_factor_v(kFarTol / err)), 0.2f), 1.0f);
+            if (h < 1e-6f) {
+                next = smin(1e-6, 5.0 * smax(fabs(static_cast<double>(y5.v[0])), 1.0));
+                done = true;
+            }
+        }
+        if (done) {
+            for (int i = 0; i < 5; ++i) s.y.v[i] = static_cast<double>(y5.v[i]);
+            return next;
+        }
+    }
+    return h;   // not reached
 }
 
 // Classic RK4 step (include/grrt/geodesic/rk4.h:34-52).
@@ -305,7 +414,8 @@ __global__ __launch_bounds__(256) void trace_vol(const TraceParams* Pg) {
                 }
             }
         }
-        dl = adaptive_dp45(K, s, dl, P.tol, attempts);
+        if (kFarFieldF32<RT> && P.far_f32 && r > kFarField * Vw.M) dl = adaptive_dp45_f32(K, s, dl, attempts);
+        else dl = adaptive_dp45(K, s, dl, P.tol, attempts);
         ++steps;
         bool opaque = false;
         if (!kSpectral) {

@@ -112,7 +112,11 @@ GRRT_EXPORT int grrt_ext_autotune(GRRTContext* ctx, const GRRTView* views, int n
 
 /* Volumetric path precision: 0 = all FP64 (default), 1 = FP32 radiative
  * transfer (opacities, source, extinction) with compensated summation of
- * each bin's intensity; the geodesic stays FP64. */
+ * each bin's intensity, the geodesic in FP64; 2 = as 1, and the geodesic's
+ * outer adaptive steps in FP32 (tolerance 1e-6) while r > 15 M.  Mode 2
+ * moves each ray's disk entry by ~1e-5 M, which shifts the coarse
+ * ray-march's step sequence: per-pixel intensities then differ from the
+ * all-FP64 result by the ray-march's own discretisation error. */
 GRRT_EXPORT int grrt_ext_set_precision(GRRTContext* ctx, int mode);
 /* grrt_render_spectral into a float32 cube (width*height*nbins). */
 GRRT_EXPORT int grrt_ext_render_spectral_f32(GRRTContext* ctx, float* out, int width, int height);
