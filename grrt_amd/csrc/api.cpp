@@ -249,7 +249,8 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
             upload(ctx->nu_obs, ctx->frequency_bins, ctx->stream);
             upload(ctx->lognu_obs, lg, ctx->stream);
             const size_t rt = ctx->mixed_precision ? sizeof(float) : sizeof(double);
-            check(ctx->rt_scratch.ensure(4 * static_cast<size_t>(nb) * n_rays * rt), "hipMalloc(rt scratch)");
+            if (vol_group_size(nb) == 0)
+                check(ctx->rt_scratch.ensure(4 * static_cast<size_t>(nb) * n_rays * rt), "hipMalloc(rt scratch)");
             check(ctx->sample_spec.ensure(static_cast<size_t>(nb) * n_rays), "hipMalloc(spectra)");
             P.nu_obs = ctx->nu_obs.ptr;
             P.lognu_obs = ctx->lognu_obs.ptr;
@@ -269,7 +270,7 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
                          ctx->stream), "upload params");
     check(launch_reset(P.counters, P.sample_cls, n_rays, ctx->stream), "reset kernel");
     check(hipEventRecord(ctx->ev[0], ctx->stream), "hipEventRecord");
-    if (vol) check(launch_trace_vol(ctx->params_dev.ptr, n_rays, spectral, ctx->mixed_precision, ctx->stream),
+    if (vol) check(launch_trace_vol(ctx->params_dev.ptr, n_rays, nb, spectral, ctx->mixed_precision, ctx->stream),
                    "volumetric trace launch");
     else check(launch_trace_thin(ctx->params_dev.ptr, 0, blocks, threads, ctx->stream), "trace kernel launch");
     check(hipEventRecord(ctx->ev[1], ctx->stream), "hipEventRecord");

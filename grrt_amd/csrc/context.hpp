@@ -37,7 +37,12 @@ struct DevBuf {
 };
 
 hipError_t launch_reset(DevCounters* c, uint8_t* cls, long long n_rays, hipStream_t st);
-hipError_t launch_trace_vol(const TraceParams* P, long long n_rays, bool spectral, bool fp32_rt, hipStream_t st);
+// Lanes per ray of the spectral volumetric kernel for nbins bins (a power of
+// two, kVolBinsPerLane bins per lane in registers), or 0 when nbins exceeds
+// 64 * kVolBinsPerLane and the bins live in device scratch, one lane per ray.
+int vol_group_size(int nbins);
+hipError_t launch_trace_vol(const TraceParams* P, long long n_rays, int nbins, bool spectral, bool fp32_rt,
+                            hipStream_t st);
 hipError_t launch_finalize_spectral(const double* spec, void* out, long long npix, int nS, int nb, bool f32,
                                     hipStream_t st);
 // mode: 0 one view, 1 batch of views, 2 listed rays of one view.

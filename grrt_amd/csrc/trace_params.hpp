@@ -14,6 +14,7 @@ constexpr int kSpecN = 1000;      // src/spectrum.h:27 default entries
 constexpr double kSpecTmin = 1000.0, kSpecTmax = 100000.0;
 constexpr int kStarGT = 180, kStarGP = 360;   // src/celestial_sphere.h:123-124
 constexpr int kStarCells = kStarGT * kStarGP;
+constexpr int kVolBinsPerLane = 4;  // spectral bins held in registers per lane of a ray's group
 
 // Queue and statistics counters in device memory (reset by a kernel before
 // every launch, so the launch sequence is graph-capturable).
@@ -127,7 +128,7 @@ struct TraceParams {
     int far_f32;                        // FP32 transfer path: FP32 geodesic steps outside r = 15 M
     const double* nu_obs;               // observer-frame frequencies [Hz]
     const double* lognu_obs;            // log10 of the same
-    void* rt_scratch;                   // spectral accumulators, 4 * nbins * n_rays of RT
+    void* rt_scratch;                   // spectral accumulators, 4 * nbins * n_rays of RT (bins > 64 * kVolBinsPerLane)
     double* sample_spec;                // n_rays * nbins spectral samples
 };
 

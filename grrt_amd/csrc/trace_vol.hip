@@ -230,32 +230,50 @@ __device__ GState rk4_step(const KerrK& K, const GState& s, double h) {
 
 // Per-ray radiative-transfer accumulators: J in units of 2h/c^2 (with
 // Neumaier compensation Jc when RT is float), transmission T and
-// accumulated optical depth tau.  RGB mode keeps its three channels in
-// registers; spectral mode keeps N bins in device scratch laid out
-// [quantity][bin][ray] so one bin of a wavefront is contiguous.
-template <class RT, bool kSpectral>
+// accumulated optical depth tau, addressed by slot j of this lane:
+//   kBinsRGB     three channels in registers, one lane per ray;
+//   kBinsGlobal  any number of spectral bins in device scratch laid out
+//                [quantity][bin][ray], one lane per ray;
+//   kBinsGroup   up to 64 * kVolBinsPerLane bins over a group of G lanes
+//                (G a power of two dividing the wavefront): lane b of the
+//                group holds bins b, b + G, b + 2G, ... in registers.  Every
+//                lane of the group integrates the same geodesic, so the
+//                group's lanes always take the same branches; only the bin
+//                loop is split, and the two quantities that couple bins (the
+//                step-control bin's opacity and the all-bins-opaque exit)
+//                are exchanged inside the group.
+enum : int { kBinsRGB = 0, kBinsGlobal = 1, kBinsGroup = 2 };
+
+template <class RT, int kKind>
 struct Bins;
 
 template <class RT>
-struct Bins<RT, false> {
+struct Bins<RT, kBinsRGB> {
     RT Jv[3], Jcv[3], Tv[3], tauv[3];
-    __device__ Bins(const TraceParams&, long long) {
+    __device__ Bins(const TraceParams&, long long, int) {
         for (int c = 0; c < 3; ++c) { Jv[c] = RT(0); Jcv[c] = RT(0); Tv[c] = RT(1); tauv[c] = RT(0); }
     }
-    __device__ RT& J(int c) { return Jv[c]; }
-    __device__ RT& Jc(int c) { return Jcv[c]; }
-    __device__ RT& T(int c) { return Tv[c]; }
-    __device__ RT& tau(int c) { return tauv[c]; }
+    static constexpr int slots() { return 3; }
+    __device__ int bin(int j) const { return j; }
+    __device__ bool lead() const { return true; }
+    __device__ RT share(RT v, int) const { return v; }
+    __device__ bool all_above(RT v, RT lim) const { return v > lim; }
+    __device__ RT& J(int j) { return Jv[j]; }
+    __device__ RT& Jc(int j) { return Jcv[j]; }
+    __device__ RT& T(int j) { return Tv[j]; }
+    __device__ RT& tau(int j) { return tauv[j]; }
 };
 
 template <class RT>
-struct Bins<RT, true> {
+struct Bins<RT, kBinsGlobal> {
     RT *Jp, *Jcp, *Tp, *taup;
     long long stride;
-    __device__ Bins(const TraceParams& P, long long id) {
+    int nb;
+    __device__ Bins(const TraceParams& P, long long id, int) {
         RT* base = static_cast<RT*>(P.rt_scratch);
         const long long plane = static_cast<long long>(P.nbins) * P.n_rays;
         stride = P.n_rays;
+        nb = P.nbins;
         Jp = base + id;
         Jcp = base + plane + id;
         Tp = base + 2 * plane + id;
@@ -267,10 +285,43 @@ struct Bins<RT, true> {
             taup[c * stride] = RT(0);
         }
     }
-    __device__ RT& J(int c) { return Jp[c * stride]; }
-    __device__ RT& Jc(int c) { return Jcp[c * stride]; }
-    __device__ RT& T(int c) { return Tp[c * stride]; }
-    __device__ RT& tau(int c) { return taup[c * stride]; }
+    __device__ int slots() const { return nb; }
+    __device__ int bin(int j) const { return j; }
+    __device__ bool lead() const { return true; }
+    __device__ RT share(RT v, int) const { return v; }
+    __device__ bool all_above(RT v, RT lim) const { return v > lim; }
+    __device__ RT& J(int j) { return Jp[j * stride]; }
+    __device__ RT& Jc(int j) { return Jcp[j * stride]; }
+    __device__ RT& T(int j) { return Tp[j * stride]; }
+    __device__ RT& tau(int j) { return taup[j * stride]; }
+};
+
+template <class RT>
+struct Bins<RT, kBinsGroup> {
+    RT Jv[kVolBinsPerLane], Jcv[kVolBinsPerLane], Tv[kVolBinsPerLane], tauv[kVolBinsPerLane];
+    int b, G, nb;
+    unsigned long long gbits;   // the group's lanes in a wavefront-wide mask
+    __device__ Bins(const TraceParams& P, long long, int group) : G(group), nb(P.nbins) {
+        const int lane = static_cast<int>(threadIdx.x & 63u);
+        b = lane & (G - 1);
+        gbits = (G == 64 ? ~0ull : ((1ull << G) - 1ull)) << (lane - b);
+#pragma unroll kVolBinsPerLane
+        for (int j = 0; j < kVolBinsPerLane; ++j) { Jv[j] = RT(0); Jcv[j] = RT(0); Tv[j] = RT(1); tauv[j] = RT(0); }
+    }
+    static constexpr int slots() { return kVolBinsPerLane; }
+    __device__ int bin(int j) const {
+        const int c = j * G + b;
+        return c < nb ? c : -1;
+    }
+    __device__ bool lead() const { return b == 0; }
+    // The value held by the lane that owns bin c.
+    __device__ RT share(RT v, int c) const { return __shfl(v, c & (G - 1), G); }
+    // v > lim on every lane of the group.
+    __device__ bool all_above(RT v, RT lim) const { return (__ballot(!(v > lim)) & gbits) == 0ull; }
+    __device__ RT& J(int j) { return Jv[j]; }
+    __device__ RT& Jc(int j) { return Jcv[j]; }
+    __device__ RT& T(int j) { return Tv[j]; }
+    __device__ RT& tau(int j) { return tauv[j]; }
 };
 
 // Compensated addition (Neumaier 1974): s + c carries the exact running sum.
@@ -289,12 +340,12 @@ struct MarchOut {
 // Backward ray-march through the volume from `st`; advances `st` to the exit
 // state.  RGB mode: channels at 450/550/650 nm, step control on 550 nm and
 // an opacity exit; spectral mode: step control on bin N/2 and an exit once
-// every bin is past tau = 10.
-template <class RT, bool kSpectral>
+// every bin is past tau = 10.  Each lane updates the bins of its slots.
+template <class RT, int kKind>
 __device__ MarchOut raymarch(const TraceParams& P, const int* perm, const KerrK& K, GState& st,
-                             Bins<RT, kSpectral>& b) {
+                             Bins<RT, kKind>& b) {
+    constexpr bool kSpectral = kKind != kBinsRGB;
     const VolDev& V = P.vol;
-    const int nb = kSpectral ? P.nbins : 3;
     const int med = kSpectral ? P.nbins / 2 : 1;
     const double ut_obs = 1.0 / sqrt(1.0 - 2.0 / P.view.r_obs);
     MarchOut mo{0, 0};
@@ -305,8 +356,11 @@ __device__ MarchOut raymarch(const TraceParams& P, const int* perm, const KerrK&
     double ds = been_inside ? H0 / 16.0 : smin(__builtin_fabs(z0) / 8.0, H0 * 2.0);
     // Exit tests use the bins' running extreme values.
     RT ext = kSpectral ? RT(1e30) : RT(0);
-    for (int c = 0; c < nb; ++c)
-        ext = kSpectral ? (b.tau(c) < ext ? b.tau(c) : ext) : (b.T(c) > ext ? b.T(c) : ext);
+#pragma unroll kVolBinsPerLane
+    for (int j = 0; j < b.slots(); ++j) {
+        if (b.bin(j) < 0) continue;
+        ext = kSpectral ? (b.tau(j) < ext ? b.tau(j) : ext) : (b.T(j) > ext ? b.T(j) : ext);
+    }
     for (int n = 0; n < 4096; ++n) {
         const GState ns = rk4_step(K, st, ds);
         ++mo.steps;
@@ -314,7 +368,7 @@ __device__ MarchOut raymarch(const TraceParams& P, const int* perm, const KerrK&
         const double th = ns.y.v[1], ph = ns.y.v[2];
         const double z = r * cos(th);
         if (r < V.r_horizon || r > V.r_outer) { st = ns; break; }
-        if (kSpectral ? (ext > RT(10)) : (ext < RT(1e-6))) { st = ns; break; }
+        if (kSpectral ? b.all_above(ext, RT(10)) : (ext < RT(1e-6))) { st = ns; break; }
         const double H = vol_H(V, r);
         if (!vol_inside(V, r, z)) {
             const double zm = vol_zmax(V, r);
@@ -340,7 +394,10 @@ __device__ MarchOut raymarch(const TraceParams& P, const int* perm, const KerrK&
         const RT rhoT = static_cast<RT>(rho), TT = static_cast<RT>(Tmp);
         RT kmed = RT(0);
         ext = kSpectral ? RT(1e30) : RT(0);
-        for (int c = 0; c < nb; ++c) {
+#pragma unroll kVolBinsPerLane
+        for (int j = 0; j < b.slots(); ++j) {
+            const int c = b.bin(j);
+            if (c < 0) continue;
             const double nu_o = kSpectral ? P.nu_obs[c] : (c == 0 ? 2.997924e10 / 450e-7 : (c == 1 ? 2.997924e10 / 550e-7 : 2.997924e10 / 650e-7));
             const double lnu_o = kSpectral ? P.lognu_obs[c] : log10(nu_o);
             const RT nue = static_cast<RT>(ag * nu_o);
@@ -351,17 +408,18 @@ __device__ MarchOut raymarch(const TraceParams& P, const int* perm, const KerrK&
             const RT dtau = ktot * rhoT * dsp;
             const RT src = eps * vol_source(nue, TT);   // S = eps B_nu(nu_e) / nu_e^3 in units of 2h/c^2
             const RT ex = exp(-dtau);
-            RT& Tc = b.T(c);
-            neumaier(b.J(c), b.Jc(c), Tc * src * (RT(1) - ex));
+            RT& Tc = b.T(j);
+            neumaier(b.J(j), b.Jc(j), Tc * src * (RT(1) - ex));
             Tc *= ex;
             if (kSpectral) {
-                RT& tc = b.tau(c);
+                RT& tc = b.tau(j);
                 tc += dtau;
                 ext = tc < ext ? tc : ext;
             } else {
                 ext = Tc > ext ? Tc : ext;
             }
         }
+        kmed = b.share(kmed, med);
         const double alpha_tot = static_cast<double>((kmed + kes) * rhoT);
         const double ds_tau = alpha_tot > 0.0 ? 0.05 / alpha_tot : ds * 2.0;
         const double ds_geo = 0.1 * smax(r - V.r_horizon, 0.5);
@@ -372,26 +430,42 @@ __device__ MarchOut raymarch(const TraceParams& P, const int* perm, const KerrK&
     return mo;
 }
 
+// Launch order -> ray index (row-major samples of the window): whole rows
+// centre-out, the middle row first and then alternately the rows above and
+// below it, so the rays that cross the disk, the longest ones, start first
+// instead of holding the launch open at its end.
+__device__ __forceinline__ long long centre_out_ray(const ViewParams& V, long long q) {
+    const long long row_rays = static_cast<long long>(V.tw) * V.nS;
+    const long long k = q / row_rays;
+    const long long mid = V.th / 2, d = (k + 1) / 2;
+    const long long row = (k & 1) ? mid - d : mid + d;
+    return row * row_rays + (q - k * row_rays);
+}
+
 }  // namespace
 
-// One lane per ray: outer Dormand–Prince loop with the near-disk step clamp
-// and volume entry detection, ray-marching on every entry.
-template <class RT, bool kSpectral>
-__global__ __launch_bounds__(256) void trace_vol(const TraceParams* Pg) {
+// Outer Dormand–Prince loop with the near-disk step clamp and volume entry
+// detection, ray-marching on every entry.  One lane per ray, or for
+// kBinsGroup `group` lanes per ray (each with its share of the bins).
+template <class RT, int kKind>
+__global__ __launch_bounds__(256) void trace_vol(const TraceParams* Pg, int group) {
+    constexpr bool kSpectral = kKind != kBinsRGB;
     __shared__ int s_perm[512];
     for (int i = threadIdx.x; i < 512; i += blockDim.x) s_perm[i] = Pg->vol.perm[i];
     __syncthreads();
     const TraceParams& P = *Pg;
-    const long long id = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (id >= P.n_rays) return;
+    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const long long q = kKind == kBinsGroup ? t / group : t;
+    if (q >= P.n_rays) return;
     const VolDev& V = P.vol;
     const ViewParams& Vw = P.view;
+    const long long id = centre_out_ray(Vw, q);
     const KerrK K{Vw.M, Vw.a, Vw.a * Vw.a, 2.0 * Vw.M};
     GState s;
     SampleId sid;
     decode_ray(Vw, id, sid);      // row-major order: every id is a sample of the window
     camera_ray(Vw, sid, s.P0, s.P3, s.y);
-    Bins<RT, kSpectral> b(P, id);
+    Bins<RT, kKind> b(P, id, group);
     const int nb = kSpectral ? P.nbins : 3;
     const double half_pi = kPi / 2.0;
     double dl = 0.01 * Vw.r_obs;
@@ -436,7 +510,7 @@ __global__ __launch_bounds__(256) void trace_vol(const TraceParams* Pg) {
                 const double rlo = smin(rp, rn), rhi = smax(rp, rn);
                 if (rhi >= V.r_horizon && rlo <= V.r_outer && rp >= V.r_horizon * 0.9 && rp <= V.r_outer * 1.5) {
                     GState entry = prev;
-                    const MarchOut mo = raymarch<RT, kSpectral>(P, s_perm, K, entry, b);
+                    const MarchOut mo = raymarch<RT, kKind>(P, s_perm, K, entry, b);
                     emitted |= mo.emitted;
                     march_steps += mo.steps;
                     ++entries;
@@ -449,9 +523,12 @@ __global__ __launch_bounds__(256) void trace_vol(const TraceParams* Pg) {
     // I_obs = J nu_obs^3 (2h/c^2 restores CGS units).
     constexpr double two_h_c2 = 2.0 * 6.626070e-27 / (2.997924e10 * 2.997924e10);
     double col[3] = {0.0, 0.0, 0.0};
-    for (int c = 0; c < nb; ++c) {
+#pragma unroll kVolBinsPerLane
+    for (int j = 0; j < b.slots(); ++j) {
+        const int c = b.bin(j);
+        if (c < 0) continue;
         const double nu_o = kSpectral ? P.nu_obs[c] : (c == 0 ? 2.997924e10 / 450e-7 : (c == 1 ? 2.997924e10 / 550e-7 : 2.997924e10 / 650e-7));
-        const double Jt = static_cast<double>(b.J(c)) + static_cast<double>(b.Jc(c));
+        const double Jt = static_cast<double>(b.J(j)) + static_cast<double>(b.Jc(j));
         const double I = Jt * two_h_c2 * nu_o * nu_o * nu_o;
         if (kSpectral) P.sample_spec[id * nb + c] = I;
         else col[c] = I;
@@ -464,6 +541,7 @@ __global__ __launch_bounds__(256) void trace_vol(const TraceParams* Pg) {
         }
         for (int c = 0; c < 3; ++c) P.sample_rgb[3 * id + c] = col[c];
     }
+    if (!b.lead()) return;          // per-ray outputs: one lane of the group
     const int cls = emitted ? GRRT_CLASS_DISK
                             : (term == 0 ? GRRT_CLASS_CAPTURED : (term == 1 ? GRRT_CLASS_ESCAPED : GRRT_CLASS_MAXSTEPS));
     P.sample_cls[id] = static_cast<uint8_t>(cls);
@@ -508,16 +586,28 @@ __global__ void finalize_spectral(const double* __restrict__ spec, void* out, lo
     else static_cast<double*>(out)[q] = acc * inv;
 }
 
-hipError_t launch_trace_vol(const TraceParams* P, long long n_rays, bool spectral, bool fp32_rt,
+int vol_group_size(int nbins) {
+    if (nbins > 64 * kVolBinsPerLane) return 0;
+    int g = 1;
+    while (g * kVolBinsPerLane < nbins) g *= 2;
+    return g;
+}
+
+hipError_t launch_trace_vol(const TraceParams* P, long long n_rays, int nbins, bool spectral, bool fp32_rt,
                             hipStream_t st) {
     const int T = 256;
-    const dim3 grid(static_cast<unsigned>((n_rays + T - 1) / T));
-    if (spectral) {
-        if (fp32_rt) hipLaunchKernelGGL((trace_vol<float, true>), grid, dim3(T), 0, st, P);
-        else hipLaunchKernelGGL((trace_vol<double, true>), grid, dim3(T), 0, st, P);
+    const int G = spectral ? vol_group_size(nbins) : 1;
+    const long long threads = n_rays * (G > 0 ? G : 1);
+    const dim3 grid(static_cast<unsigned>((threads + T - 1) / T));
+    if (spectral && G > 0) {
+        if (fp32_rt) hipLaunchKernelGGL((trace_vol<float, kBinsGroup>), grid, dim3(T), 0, st, P, G);
+        else hipLaunchKernelGGL((trace_vol<double, kBinsGroup>), grid, dim3(T), 0, st, P, G);
+    } else if (spectral) {
+        if (fp32_rt) hipLaunchKernelGGL((trace_vol<float, kBinsGlobal>), grid, dim3(T), 0, st, P, 1);
+        else hipLaunchKernelGGL((trace_vol<double, kBinsGlobal>), grid, dim3(T), 0, st, P, 1);
     } else {
-        if (fp32_rt) hipLaunchKernelGGL((trace_vol<float, false>), grid, dim3(T), 0, st, P);
-        else hipLaunchKernelGGL((trace_vol<double, false>), grid, dim3(T), 0, st, P);
+        if (fp32_rt) hipLaunchKernelGGL((trace_vol<float, kBinsRGB>), grid, dim3(T), 0, st, P, 1);
+        else hipLaunchKernelGGL((trace_vol<double, kBinsRGB>), grid, dim3(T), 0, st, P, 1);
     }
     return hipGetLastError();
 }

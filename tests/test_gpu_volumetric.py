@@ -187,3 +187,28 @@ def test_spectral_fits_stream_equals_cube(grrt, oracle, tmp_path):
         n = W * H * len(freqs)
         data = np.frombuffer(raw, dtype=dt, count=n, offset=hdr).reshape(len(freqs), H, W)
         assert np.array_equal(data[:, ::-1, :].transpose(1, 2, 0), ref)
+
+
+def test_grouped_bins_equal_lane_per_ray(grrt, oracle):
+    """The spectral kernel holds up to 256 bins in registers, spread over a
+    group of lanes per ray; longer lists run one lane per ray with the bins in
+    device scratch.  A 257-bin list made only of the 32 C3 frequencies, with
+    the step-control bin N/2 on the same frequency, takes the second path and
+    must give every copy of each frequency bit for bit the 32-bin grouped
+    result (all-FP64 and FP32-transfer modes)."""
+    print("case: grouped vs lane-per-ray bins, 40x40, 32 vs 257 bins", flush=True)
+    W = 40
+    p, _, _ = _setup(grrt, oracle, W, W, freqs=FREQS)
+    idx = list(range(16, 32)) + list(range(32)) * 7 + list(range(16)) + [16]
+    assert len(idx) == 257 and idx[257 // 2] == 32 // 2
+    big = [FREQS[i] for i in idx]
+    with grrt.Context(p) as ctx:
+        for mode in (0, 1):
+            ctx.set_precision(mode)
+            ctx.set_frequency_bins(FREQS)
+            a = ctx.render_spectral() if mode == 0 else ctx.render_spectral_f32()
+            ctx.set_frequency_bins(big)
+            b = ctx.render_spectral() if mode == 0 else ctx.render_spectral_f32()
+            assert (a.sum(-1) > 0).mean() > 0.05
+            for k, i in enumerate(idx):
+                assert np.array_equal(b[..., k], a[..., i]), (mode, k, i)
