@@ -595,8 +595,13 @@ int vol_group_size(int nbins) {
 
 hipError_t launch_trace_vol(const TraceParams* P, long long n_rays, int nbins, bool spectral, bool fp32_rt,
                             hipStream_t st) {
-    const int T = 256;
     const int G = spectral ? vol_group_size(nbins) : 1;
+    // Grouped spectral rays: one wavefront per block, so a block is
+    // dispatched as soon as one wave slot frees and the long-running rays of
+    // the disk region do not hold back whole 4-wave blocks (C3: 42 ms against
+    // 89 ms with 256 threads).  One lane per ray: 256 threads (the RGB frame
+    // of the same scene: 172 ms against 199 ms with 64).
+    const int T = (spectral && G > 0) ? 64 : 256;
     const long long threads = n_rays * (G > 0 ? G : 1);
     const dim3 grid(static_cast<unsigned>((threads + T - 1) / T));
     if (spectral && G > 0) {
