@@ -255,6 +255,72 @@ def c4_section(grrt, reps: int) -> dict:
             "batch_speedup": round(seq_ms / batch_ms, 3)}
 
 
+C3_FREQS = [10 ** (14 + 4 * k / 31) for k in range(32)]
+
+
+def c3_params(grrt):
+    return grrt.make_params(width=512, height=512, metric=grrt.METRIC_KERR, spin=0.9,
+                            observer_theta=math.radians(75.0), fov=math.radians(90.0), volumetric=True,
+                            disk_outer=20.0, turbulence=0.4, seed=42, octaves=2, alpha=0.1,
+                            freqs=C3_FREQS)
+
+
+def c3_section(grrt, reps: int, cpu: bool) -> dict:
+    """The volumetric spectral cube: Kerr a=0.9, 512^2 x 1 spp (pixel-hash
+    jitter), theta=75 deg, alpha=0.1, turbulence 0.4, 2 octaves, seed 42,
+    32 frequencies log-spaced over 1e14-1e18 Hz, float32 cube; timed in the
+    mixed form (FP32 transfer, FP64 geodesic) and all-FP64, with the CPU
+    oracle (all FP64) on a random sample of the same rays."""
+    p = c3_params(grrt)
+    t0 = time.perf_counter()
+    ctx = grrt.Context(p)
+    setup_s = time.perf_counter() - t0
+    ctx.set_frequency_bins(C3_FREQS)
+    rays = 512 * 512
+    out = {"workload": "Kerr a=0.9 512^2 x 1 spp, theta=75deg, volumetric alpha=0.1 turb 0.4 2 oct seed 42, "
+                       "32 bins 1e14-1e18 Hz, float32 cube",
+           "context_setup_s": round(setup_s, 2)}
+    for mode, name in ((1, "mixed"), (0, "fp64")):
+        ctx.set_precision(mode)
+        ctx.render_spectral_f32()
+        wall, kern = [], []
+        for _ in range(reps):
+            t1 = time.perf_counter()
+            ctx.render_spectral_f32()
+            wall.append(time.perf_counter() - t1)
+            kern.append(ctx.stats().trace_ms)
+        ms = sum(wall) / reps * 1e3
+        out[name] = {"ms_per_cube": round(ms, 2), "trace_kernel_ms": round(sum(kern) / reps, 2),
+                     "mrays_s": round(rays / (ms / 1e3) / 1e6, 3),
+                     "attempts_per_cube": int(ctx.stats().attempts)}
+    ctx.close()
+    if cpu:
+        import numpy as np
+        import oracle as O
+        T = grrt.volumetric_tables(p)
+        V = O.vol_struct(T)
+        sc = O.scene(width=512, height=512, spin_a=0.9, theta_obs=math.radians(75.0), fov=math.radians(90.0),
+                     disk=False)
+        threads = max(1, min(16, os.cpu_count() or 1))
+        rng = np.random.default_rng(2025)
+        probe = np.sort(rng.choice(rays, 1000, replace=False))
+        t1 = time.perf_counter()
+        O.vol_trace_samples(sc, V, O.JITTER_HASH, 1, 0, probe, freqs=C3_FREQS, threads=threads, variant="fast")
+        rate = len(probe) / (time.perf_counter() - t1)
+        n = int(min(rays, max(1000, rate * 10.0)))
+        ids = np.sort(rng.choice(rays, n, replace=False))
+        t1 = time.perf_counter()
+        O.vol_trace_samples(sc, V, O.JITTER_HASH, 1, 0, ids, freqs=C3_FREQS, threads=threads, variant="fast")
+        dt = time.perf_counter() - t1
+        cube_s = rays / (n / dt)
+        out["cpu_baseline"] = {"value": round(n / dt / 1e6, 5), "unit": "Mrays/s", "cores": threads,
+                               "kind": "port", "cube_s": round(cube_s, 2),
+                               "sample": f"{n} rays drawn uniformly from the cube's 512^2, {dt:.1f} s, oracle "
+                                         f"built -O3 -ffast-math, all FP64, OpenMP guided"}
+        out["speedup_mixed_vs_cpu"] = round(cube_s * 1e3 / out["mixed"]["ms_per_cube"], 1)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -266,6 +332,7 @@ def main() -> int:
     ap.add_argument("--sg-steps", type=int, default=60000, help="surrogate training steps")
     ap.add_argument("--c5-frames", type=int, default=2, help="2048^2 frames timed per form (0: skip)")
     ap.add_argument("--c4-reps", type=int, default=2, help="replays of the 64-view batch (0: skip)")
+    ap.add_argument("--c3-reps", type=int, default=3, help="volumetric spectral cubes timed per form (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
@@ -334,6 +401,8 @@ def main() -> int:
     ctx.close()
     if args.c4_reps > 0:
         out["batched_views"] = c4_section(grrt, args.c4_reps)
+    if args.c3_reps > 0:
+        out["volumetric_spectral"] = c3_section(grrt, args.c3_reps, not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
