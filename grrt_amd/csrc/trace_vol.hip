@@ -230,39 +230,33 @@ __device__ GState rk4_step(const KerrK& K, const GState& s, double h) {
 
 // Per-ray radiative-transfer accumulators: J in units of 2h/c^2 (with
 // Neumaier compensation Jc when RT is float), transmission T and
-// accumulated optical depth tau, addressed by slot j of this lane:
-//   kBinsRGB     three channels in registers, one lane per ray;
+// accumulated optical depth tau, addressed by slot j of this lane.  The
+// kernel kinds:
+//   kBinsRGB     three channels (450/550/650 nm) over a group of kRgbGroup
+//                lanes per ray, channel c on lane c of the group;
 //   kBinsGlobal  any number of spectral bins in device scratch laid out
 //                [quantity][bin][ray], one lane per ray;
-//   kBinsGroup   up to 64 * kVolBinsPerLane bins over a group of G lanes
-//                (G a power of two dividing the wavefront): lane b of the
-//                group holds bins b, b + G, b + 2G, ... in registers.  Every
-//                lane of the group integrates the same geodesic, so the
-//                group's lanes always take the same branches; only the bin
-//                loop is split, and the two quantities that couple bins (the
-//                step-control bin's opacity and the all-bins-opaque exit)
-//                are exchanged inside the group.
+//   kBinsGroup   up to 64 * kVolBinsPerLane spectral bins over a group of G
+//                lanes (G a power of two dividing the wavefront): lane b of
+//                the group holds bins b, b + G, b + 2G, ... in registers.
+// Every lane of a group integrates the same geodesic, so the group's lanes
+// always take the same branches; only the bin loop is split, and the
+// quantities that couple bins (the step-control bin's opacity, the
+// all-bins-opaque exits, the record's colour) are exchanged inside the
+// group.  A few rays per wavefront keep the rays of one wavefront from
+// waiting on one another's long ray-marches (the RGB frame of the C3 scene:
+// 172 ms with one lane per ray, 63 ms with 8 lanes, 58 ms with 16).
 enum : int { kBinsRGB = 0, kBinsGlobal = 1, kBinsGroup = 2 };
+constexpr int kRgbGroup = 16;
+static_assert(kRgbGroup >= 3 && kRgbGroup <= 64 && (kRgbGroup & (kRgbGroup - 1)) == 0,
+              "each RGB channel is slot 0 of its own lane");
 
-template <class RT, int kKind>
+template <class RT, int kStore>
 struct Bins;
 
-template <class RT>
-struct Bins<RT, kBinsRGB> {
-    RT Jv[3], Jcv[3], Tv[3], tauv[3];
-    __device__ Bins(const TraceParams&, long long, int) {
-        for (int c = 0; c < 3; ++c) { Jv[c] = RT(0); Jcv[c] = RT(0); Tv[c] = RT(1); tauv[c] = RT(0); }
-    }
-    static constexpr int slots() { return 3; }
-    __device__ int bin(int j) const { return j; }
-    __device__ bool lead() const { return true; }
-    __device__ RT share(RT v, int) const { return v; }
-    __device__ bool all_above(RT v, RT lim) const { return v > lim; }
-    __device__ RT& J(int j) { return Jv[j]; }
-    __device__ RT& Jc(int j) { return Jcv[j]; }
-    __device__ RT& T(int j) { return Tv[j]; }
-    __device__ RT& tau(int j) { return tauv[j]; }
-};
+// Storage of a kernel kind: scratch for kBinsGlobal, lane groups otherwise.
+template <class RT, int kKind>
+using BinsOf = Bins<RT, kKind == kBinsGlobal ? kBinsGlobal : kBinsGroup>;
 
 template <class RT>
 struct Bins<RT, kBinsGlobal> {
@@ -288,8 +282,9 @@ struct Bins<RT, kBinsGlobal> {
     __device__ int slots() const { return nb; }
     __device__ int bin(int j) const { return j; }
     __device__ bool lead() const { return true; }
-    __device__ RT share(RT v, int) const { return v; }
-    __device__ bool all_above(RT v, RT lim) const { return v > lim; }
+    template <class V>
+    __device__ V share(V v, int) const { return v; }
+    __device__ bool all(bool pred) const { return pred; }
     __device__ RT& J(int j) { return Jp[j * stride]; }
     __device__ RT& Jc(int j) { return Jcp[j * stride]; }
     __device__ RT& T(int j) { return Tp[j * stride]; }
@@ -315,9 +310,10 @@ struct Bins<RT, kBinsGroup> {
     }
     __device__ bool lead() const { return b == 0; }
     // The value held by the lane that owns bin c.
-    __device__ RT share(RT v, int c) const { return __shfl(v, c & (G - 1), G); }
-    // v > lim on every lane of the group.
-    __device__ bool all_above(RT v, RT lim) const { return (__ballot(!(v > lim)) & gbits) == 0ull; }
+    template <class V>
+    __device__ V share(V v, int c) const { return __shfl(v, c & (G - 1), G); }
+    // pred on every lane of the group.
+    __device__ bool all(bool pred) const { return (__ballot(!pred) & gbits) == 0ull; }
     __device__ RT& J(int j) { return Jv[j]; }
     __device__ RT& Jc(int j) { return Jcv[j]; }
     __device__ RT& T(int j) { return Tv[j]; }
@@ -343,7 +339,7 @@ struct MarchOut {
 // every bin is past tau = 10.  Each lane updates the bins of its slots.
 template <class RT, int kKind>
 __device__ MarchOut raymarch(const TraceParams& P, const int* perm, const KerrK& K, GState& st,
-                             Bins<RT, kKind>& b) {
+                             BinsOf<RT, kKind>& b, double lnu_rgb) {
     constexpr bool kSpectral = kKind != kBinsRGB;
     const VolDev& V = P.vol;
     const int med = kSpectral ? P.nbins / 2 : 1;
@@ -368,7 +364,7 @@ __device__ MarchOut raymarch(const TraceParams& P, const int* perm, const KerrK&
         const double th = ns.y.v[1], ph = ns.y.v[2];
         const double z = r * cos(th);
         if (r < V.r_horizon || r > V.r_outer) { st = ns; break; }
-        if (kSpectral ? b.all_above(ext, RT(10)) : (ext < RT(1e-6))) { st = ns; break; }
+        if (b.all(kSpectral ? (ext > RT(10)) : (ext < RT(1e-6)))) { st = ns; break; }
         const double H = vol_H(V, r);
         if (!vol_inside(V, r, z)) {
             const double zm = vol_zmax(V, r);
@@ -399,7 +395,7 @@ __device__ MarchOut raymarch(const TraceParams& P, const int* perm, const KerrK&
             const int c = b.bin(j);
             if (c < 0) continue;
             const double nu_o = kSpectral ? P.nu_obs[c] : (c == 0 ? 2.997924e10 / 450e-7 : (c == 1 ? 2.997924e10 / 550e-7 : 2.997924e10 / 650e-7));
-            const double lnu_o = kSpectral ? P.lognu_obs[c] : log10(nu_o);
+            const double lnu_o = kSpectral ? P.lognu_obs[c] : lnu_rgb;   // RGB: slot 0 only
             const RT nue = static_cast<RT>(ag * nu_o);
             const RT kabs = vol_kabs<RT>(V, cell, lag + lnu_o);
             if (c == med) kmed = kabs;
@@ -455,7 +451,7 @@ __global__ __launch_bounds__(256) void trace_vol(const TraceParams* Pg, int grou
     __syncthreads();
     const TraceParams& P = *Pg;
     const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const long long q = kKind == kBinsGroup ? t / group : t;
+    const long long q = kKind != kBinsGlobal ? t / group : t;
     if (q >= P.n_rays) return;
     const VolDev& V = P.vol;
     const ViewParams& Vw = P.view;
@@ -465,7 +461,13 @@ __global__ __launch_bounds__(256) void trace_vol(const TraceParams* Pg, int grou
     SampleId sid;
     decode_ray(Vw, id, sid);      // row-major order: every id is a sample of the window
     camera_ray(Vw, sid, s.P0, s.P3, s.y);
-    Bins<RT, kKind> b(P, id, group);
+    BinsOf<RT, kKind> b(P, id, group);
+    // RGB: log10 of this lane's channel frequency, once per ray.
+    double lnu_rgb = 0.0;
+    if (!kSpectral && b.bin(0) >= 0) {
+        const int c = b.bin(0);
+        lnu_rgb = log10(c == 0 ? 2.997924e10 / 450e-7 : (c == 1 ? 2.997924e10 / 550e-7 : 2.997924e10 / 650e-7));
+    }
     const int nb = kSpectral ? P.nbins : 3;
     const double half_pi = kPi / 2.0;
     double dl = 0.01 * Vw.r_obs;
@@ -493,8 +495,11 @@ __global__ __launch_bounds__(256) void trace_vol(const TraceParams* Pg, int grou
         ++steps;
         bool opaque = false;
         if (!kSpectral) {
-            opaque = true;
-            for (int c = 0; c < 3; ++c) opaque = opaque && (b.T(c) < RT(1e-6));
+            bool mine = true;
+#pragma unroll kVolBinsPerLane
+            for (int j = 0; j < b.slots(); ++j)
+                if (b.bin(j) >= 0) mine = mine && (b.T(j) < RT(1e-6));
+            opaque = b.all(mine);
         }
         if (!opaque) {
             const double tp = prev.y.v[1], tn = s.y.v[1];
@@ -510,7 +515,7 @@ __global__ __launch_bounds__(256) void trace_vol(const TraceParams* Pg, int grou
                 const double rlo = smin(rp, rn), rhi = smax(rp, rn);
                 if (rhi >= V.r_horizon && rlo <= V.r_outer && rp >= V.r_horizon * 0.9 && rp <= V.r_outer * 1.5) {
                     GState entry = prev;
-                    const MarchOut mo = raymarch<RT, kKind>(P, s_perm, K, entry, b);
+                    const MarchOut mo = raymarch<RT, kKind>(P, s_perm, K, entry, b, lnu_rgb);
                     emitted |= mo.emitted;
                     march_steps += mo.steps;
                     ++entries;
@@ -523,6 +528,7 @@ __global__ __launch_bounds__(256) void trace_vol(const TraceParams* Pg, int grou
     // I_obs = J nu_obs^3 (2h/c^2 restores CGS units).
     constexpr double two_h_c2 = 2.0 * 6.626070e-27 / (2.997924e10 * 2.997924e10);
     double col[3] = {0.0, 0.0, 0.0};
+    double mine = 0.0;               // RGB: this lane's channel (slot 0 of lanes 0-2)
 #pragma unroll kVolBinsPerLane
     for (int j = 0; j < b.slots(); ++j) {
         const int c = b.bin(j);
@@ -531,15 +537,16 @@ __global__ __launch_bounds__(256) void trace_vol(const TraceParams* Pg, int grou
         const double Jt = static_cast<double>(b.J(j)) + static_cast<double>(b.Jc(j));
         const double I = Jt * two_h_c2 * nu_o * nu_o * nu_o;
         if (kSpectral) P.sample_spec[id * nb + c] = I;
-        else col[c] = I;
+        else mine = I;
     }
     if (!kSpectral) {
-        if (term == 1 && P.stars) {
+        const int c = b.bin(0);
+        if (c >= 0) {
             // stars: the thin tracer's catalogue search (src/renderer.cpp:56-61)
-            const double bgt = star_sample(P, s.y.v[1], s.y.v[2]);
-            for (int c = 0; c < 3; ++c) col[c] += bgt;
+            if (term == 1 && P.stars) mine += star_sample(P, s.y.v[1], s.y.v[2]);
+            P.sample_rgb[3 * id + c] = mine;
         }
-        for (int c = 0; c < 3; ++c) P.sample_rgb[3 * id + c] = col[c];
+        for (int k = 0; k < 3; ++k) col[k] = b.share(mine, k);
     }
     if (!b.lead()) return;          // per-ray outputs: one lane of the group
     const int cls = emitted ? GRRT_CLASS_DISK
@@ -595,13 +602,12 @@ int vol_group_size(int nbins) {
 
 hipError_t launch_trace_vol(const TraceParams* P, long long n_rays, int nbins, bool spectral, bool fp32_rt,
                             hipStream_t st) {
-    const int G = spectral ? vol_group_size(nbins) : 1;
-    // Grouped spectral rays: one wavefront per block, so a block is
-    // dispatched as soon as one wave slot frees and the long-running rays of
-    // the disk region do not hold back whole 4-wave blocks (C3: 42 ms against
-    // 89 ms with 256 threads).  One lane per ray: 256 threads (the RGB frame
-    // of the same scene: 172 ms against 199 ms with 64).
-    const int T = (spectral && G > 0) ? 64 : 256;
+    const int G = spectral ? vol_group_size(nbins) : kRgbGroup;
+    // Lane groups: one wavefront per block, so a block is dispatched as
+    // soon as one wave slot frees and the long-running rays of the disk
+    // region do not hold back whole 4-wave blocks (C3: 42 ms against 89 ms
+    // with 256 threads).  One lane per ray (more than 256 bins): 256 threads.
+    const int T = G > 0 ? 64 : 256;
     const long long threads = n_rays * (G > 0 ? G : 1);
     const dim3 grid(static_cast<unsigned>((threads + T - 1) / T));
     if (spectral && G > 0) {
@@ -611,8 +617,8 @@ hipError_t launch_trace_vol(const TraceParams* P, long long n_rays, int nbins, b
         if (fp32_rt) hipLaunchKernelGGL((trace_vol<float, kBinsGlobal>), grid, dim3(T), 0, st, P, 1);
         else hipLaunchKernelGGL((trace_vol<double, kBinsGlobal>), grid, dim3(T), 0, st, P, 1);
     } else {
-        if (fp32_rt) hipLaunchKernelGGL((trace_vol<float, kBinsRGB>), grid, dim3(T), 0, st, P, 1);
-        else hipLaunchKernelGGL((trace_vol<double, kBinsRGB>), grid, dim3(T), 0, st, P, 1);
+        if (fp32_rt) hipLaunchKernelGGL((trace_vol<float, kBinsRGB>), grid, dim3(T), 0, st, P, G);
+        else hipLaunchKernelGGL((trace_vol<double, kBinsRGB>), grid, dim3(T), 0, st, P, G);
     }
     return hipGetLastError();
 }
