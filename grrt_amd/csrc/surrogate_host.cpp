@@ -506,7 +506,12 @@ int grrt_ext_render_surrogate(GRRTContext* ctx, float* out, int device_out, doub
         for (int k = 1; k < nb; ++k) check(launch_reset(ctx->counters.ptr + k, nullptr, 0, st), "reset");
         check(launch_reset_counts(S.counts.ptr, 4 * nb, st), "reset counts");
         check(hipEventRecord(ctx->ev[0], st), "event");
-        const int blocks = std::max(1, ctx->num_cus * ctx->tune_blocks_per_cu);
+        // The fallback list is short and led by its longest integrations; one
+        // block per CU (one wavefront per SIMD) lets those run without a
+        // second wavefront taking half of their SIMD's issue slots (the
+        // 1024^2 x 4 frame: 3.39 ms after the network pass, 3.78 ms with
+        // the plain frame's two blocks per CU).
+        const int blocks = std::max(1, ctx->num_cus);
         for (int k = 0; k < nb; ++k) {
             InferArgs IA{};
             IA.mlp = net_args(S);
