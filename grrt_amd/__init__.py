@@ -179,6 +179,8 @@ API = [
     ("grrt_ext_surrogate_dataset", C.c_int, [_P, C.c_int, _P, _P, C.c_int64]),
     ("grrt_ext_render_surrogate", C.c_int, [_P, _P, C.c_int, C.c_double, C.c_double, C.c_double,
                                             C.POINTER(GRRTSurrogateStats)]),
+    ("grrt_ext_tonemap_device", C.c_int, [_P, _P, C.POINTER(C.c_double)]),
+    ("grrt_ext_get_frame", C.c_int, [_P, _P]),
     ("grrt_ext_render_spectral_f32", C.c_int, [_P, C.POINTER(C.c_float), C.c_int, C.c_int]),
     ("grrt_ext_vol_build", _P, [C.POINTER(GRRTParams)]),
     ("grrt_ext_vol_scalars", C.c_int, [_P, C.POINTER(C.c_double), C.c_int]),
@@ -466,6 +468,19 @@ class Context:
 
     def render_device(self, d_ptr: int | None = None) -> int:
         return self._check(self._L.grrt_ext_render_device(self._h, d_ptr), "grrt_ext_render_device")
+
+    def tonemap_device(self, d_ptr: int | None = None) -> float:
+        """Tone-map the context's device frame (or d_ptr) in place; returns
+        the kernels' GPU time in ms."""
+        ms = C.c_double()
+        self._check(self._L.grrt_ext_tonemap_device(self._h, d_ptr, C.byref(ms)), "grrt_ext_tonemap_device")
+        return ms.value
+
+    def get_frame(self) -> np.ndarray:
+        """The context's device frame as a (height, width, 4) float32 array."""
+        out = np.zeros((self.params.height, self.params.width, 4), np.float32)
+        self._check(self._L.grrt_ext_get_frame(self._h, out.ctypes.data), "grrt_ext_get_frame")
+        return out
 
     def num_samples(self) -> int:
         return int(self._L.grrt_ext_num_samples(self._h))

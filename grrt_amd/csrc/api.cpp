@@ -714,6 +714,42 @@ int grrt_ext_render_device(GRRTContext* ctx, float* d_rgba) {
     });
 }
 
+int grrt_ext_tonemap_device(GRRTContext* ctx, float* d_rgba, double* gpu_ms) {
+    return guarded(ctx, [&] {
+        const auto& r = ctx->res;
+        const long long npix = static_cast<long long>(r.W) * r.H;
+        float* dst = d_rgba;
+        if (!dst) {
+            if (!ctx->frame.ptr || ctx->frame.cap < static_cast<size_t>(4 * npix))
+                throw HipFailure("no frame in the context: render it with grrt_ext_render_device(ctx, NULL)");
+            dst = ctx->frame.ptr;
+        }
+        check(ctx->tone_part.ensure(static_cast<size_t>(grrt_amd::tonemap_blocks(npix))), "hipMalloc(tone partials)");
+        check(ctx->tone_params.ensure(1), "hipMalloc(tone params)");
+        check(hipEventRecord(ctx->ev[0], ctx->stream), "hipEventRecord");
+        check(grrt_amd::launch_tonemap(dst, npix, ctx->tone_part.ptr, ctx->tone_params.ptr, ctx->stream), "tone map launch");
+        check(hipEventRecord(ctx->ev[1], ctx->stream), "hipEventRecord");
+        check(hipEventSynchronize(ctx->ev[1]), "tone map");
+        if (gpu_ms) {
+            float ms = 0.0f;
+            check(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]), "event time");
+            *gpu_ms = ms;
+        }
+        return 0;
+    });
+}
+
+int grrt_ext_get_frame(GRRTContext* ctx, float* out) {
+    return guarded(ctx, [&] {
+        const auto& r = ctx->res;
+        const size_t n = 4 * static_cast<size_t>(r.W) * r.H;
+        if (!out) throw HipFailure("null output");
+        if (!ctx->frame.ptr || ctx->frame.cap < n) throw HipFailure("no frame in the context");
+        check(hipMemcpy(out, ctx->frame.ptr, n * sizeof(float), hipMemcpyDeviceToHost), "copy frame");
+        return 0;
+    });
+}
+
 int grrt_ext_set_tuning(GRRTContext* ctx, int block, int blocks_per_cu, int refill_min, int chunk, int tile) {
     return guarded(ctx, [&] {
         if (block > 0) {

@@ -1,5 +1,6 @@
 // GRRTContext: the state behind the opaque handle of the C ABI.
 #pragma once
+#include "tonemap.hpp"
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -94,6 +95,8 @@ struct GRRTContext {
     grrt_amd::DevBuf<uint8_t> sample_cls;
     grrt_amd::DevBuf<GRRTRayRecord> records;
     grrt_amd::DevBuf<float> frame;
+    grrt_amd::DevBuf<grrt_amd::TonePartial> tone_part;   // device tone mapping: per-block partials
+    grrt_amd::DevBuf<grrt_amd::ToneParams> tone_params;
     grrt_amd::DevBuf<grrt_amd::DevCounters> counters;
     grrt_amd::DevBuf<grrt_amd::TraceParams> params_dev;   // launch parameters
     grrt_amd::TraceParams* params_host = nullptr;          // pinned staging copy

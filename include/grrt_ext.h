@@ -69,6 +69,17 @@ GRRT_EXPORT int grrt_ext_last_stats(const GRRTContext* ctx, GRRTStats* out);
 /* Render the full frame into device memory (d_rgba: width*height*4 floats,
  * or NULL to keep it in the context).  No host copy is made. */
 GRRT_EXPORT int grrt_ext_render_device(GRRTContext* ctx, float* d_rgba);
+/* Tone-maps a frame in device memory in place with the grrt_tonemap curve
+ * (log-average exposure, extended Reinhard with the frame's white point,
+ * gamma 2.2; src/tonemapper.cpp:7-71, whose grrt_tonemap works on a host
+ * buffer): d_rgba is width*height*4 floats, or NULL for the context's own
+ * frame (grrt_ext_render_device(ctx, NULL), or a surrogate or batch-free
+ * frame kept on the device).  gpu_ms (may be NULL) receives the kernels'
+ * GPU time. */
+GRRT_EXPORT int grrt_ext_tonemap_device(GRRTContext* ctx, float* d_rgba, double* gpu_ms);
+/* Copies the context's own device frame (width*height*4 floats) to host
+ * memory. */
+GRRT_EXPORT int grrt_ext_get_frame(GRRTContext* ctx, float* out);
 /* Persistent-kernel launch shape: threads per block (multiple of 64),
  * blocks per CU, the number of idle lanes (1..64) that triggers a
  * wavefront refill, and the rays a wavefront claims from the device queue
