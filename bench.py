@@ -350,7 +350,7 @@ def main() -> int:
         ctx.set_tuning(t[0], t[1], t[2], t[3], t[4] if len(t) > 4 else -1)
     for _ in range(args.warmup):
         ctx.render_device()
-    trace_ms, attempts, steps = [], 0, 0
+    trace_ms, attempts, steps, derivs = [], 0, 0, 0
     t0 = time.perf_counter()
     for _ in range(args.steps):
         ctx.render_device()          # returns after the frame is complete
@@ -358,6 +358,7 @@ def main() -> int:
         trace_ms.append(s.trace_ms)
         attempts += s.attempts
         steps += s.steps
+        derivs += s.deriv_evals
     wall = time.perf_counter() - t0
     ms_per_step = wall / args.steps * 1e3
     rays = W * H * SPP
@@ -396,6 +397,8 @@ def main() -> int:
                                "tol 1e-8, max 10000 steps",
                    "rays_per_step": rays, "parallelism": "single GPU"},
         "geodesic_steps_per_s": round(steps / args.steps / (ms_per_step / 1e3), 1),
+        "attempts_per_s": round(attempts / args.steps / (ms_per_step / 1e3), 1),
+        "derivative_evals_per_s": round(derivs / args.steps / (ms_per_step / 1e3), 1),
         "attempts_per_launch": int(att_per_launch),
         "trace_kernel_ms": round(avg_trace_s * 1e3, 3),
         "host_copy_inclusive_ms": round(host_ms, 3),
