@@ -265,7 +265,10 @@ __device__ __forceinline__ long long wave_append(bool want, unsigned long long* 
 }
 
 // Surrogate frame pass.  Persistent blocks of 4 waves (two blocks per CU);
-// each wave owns 32 rays of a 128-ray tile.  Hidden-layer weights stream
+// each wave owns 64 rays of a 256-ray tile: their inputs are computed one
+// ray per lane, the network runs as two passes of 32 rays (the MFMA tile
+// width), and the per-ray work after it (gates, judge selection, shading)
+// is done one ray per lane again.  Hidden-layer weights stream
 // through a double-buffered 16 KB LDS window, one 32-neuron block (16
 // k-steps) at a time, shared by the block's four waves; the layer-0 and
 // output-layer weights and all biases stay in LDS.  Every finished 32-neuron
@@ -275,7 +278,7 @@ __device__ __forceinline__ long long wave_append(bool want, unsigned long long* 
 // from the view's sample position, the network, then either shading from
 // the prediction or a place in the fallback list; a hashed fraction of the
 // shaded rays is listed for the FP64 judge.
-constexpr int kInferTile = 128;
+constexpr int kInferTile = 256;              // rays per block tile: 64 per wave, two 32-ray passes
 constexpr int kChunkVec = 16 * 64;          // uint4 (8 BF16) per 16 KB weight window
 constexpr int kChunks = 3 * 8;              // hidden (layer, 32-neuron block) windows
 
@@ -361,7 +364,8 @@ __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
     __shared__ uint4 s_pack0[kPack0 / 8];
     __shared__ uint4 s_pack4[kPack4 / 8];
     __shared__ float s_bias[4 * kHid + 16];
-    __shared__ float outs[4][32][kOut];
+    __shared__ float outs[4][64][kOut];
+    __shared__ float s_x[4][64][kIn];
     __shared__ double s_flux[kFluxN];
     const MlpArgs& A = IA.mlp;
     const TraceParams& P = *IA.P;
@@ -387,13 +391,13 @@ __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
     while (tile < n_tiles) {
         if (tid == 0) s_claim = static_cast<long long>(atomicAdd(IA.tile_counter, 1ull));
         bool more_tiles = true;         // read from s_claim after the first window's barrier
-        const long long ray0 = tile * kInferTile + wave * 32;
-        float x[1][kIn];
+        const long long ray0 = tile * kInferTile + wave * 64;
         {
+            // Inputs of this lane's ray (ray0 + lane), once per 64 rays.
             // View constants are re-read per tile (cached) rather than held
             // in registers across the network.
             asm volatile("" ::: "memory");
-            long long i = ray0 + c;
+            long long i = ray0 + lane;
             if (i >= n) i = n - 1;
             SampleId d;
             decode_ray(V, visit_ray(IA, i), d);
@@ -401,8 +405,19 @@ __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
             sample_position(V, d, px, py);
             const double alpha = (px - V.W / 2.0) * V.fov / V.W;
             const double beta = (py - V.H / 2.0) * V.fov / V.W;
-            features(alpha, beta, V.a / V.M, cos(V.th_obs), V.r_obs / V.M, x[0]);
+            float xl[kIn];
+            features(alpha, beta, V.a / V.M, cos(V.th_obs), V.r_obs / V.M, xl);
+#pragma unroll
+            for (int k = 0; k < kIn; ++k) s_x[wave][lane][k] = xl[k];
+            __builtin_amdgcn_wave_barrier();
         }
+        // Two passes of the network, rays ray0 + 32 t + c; the window
+        // stream continues from one pass into the next.
+#pragma unroll 1
+        for (int t = 0; t < 2; ++t) {
+        float x[1][kIn];
+#pragma unroll
+        for (int k = 0; k < kIn; ++k) x[0][k] = s_x[wave][t * 32 + c][k];
         bf16x8 actA[16], actB[16];
         // Layer 0: one k-step per 32-neuron block.
         {
@@ -429,8 +444,8 @@ __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
         // the other buffer, run this window's block (and the SiLU of the
         // previous block), then wait for the copy and publish it.
         auto window = [&](int cidx, auto&& body) {
-            if (cidx == kChunks - 1) more_tiles = s_claim < n_tiles;
-            const bool has_next = cidx + 1 < kChunks || more_tiles;
+            if (cidx == kChunks - 1 && t == 1) more_tiles = s_claim < n_tiles;
+            const bool has_next = cidx + 1 < kChunks || t == 0 || more_tiles;
             if (has_next) load_window(chunk_src(A, cidx + 1 < kChunks ? cidx + 1 : 0), wbuf[buf ^ 1]);
             body(reinterpret_cast<const bf16x8*>(wbuf[buf]));
             __builtin_amdgcn_s_waitcnt(0);
@@ -488,17 +503,18 @@ __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             const int m = acc_row(g, v);
-            if (m < kOut) outs[wave][c][m] = o[v];
+            if (m < kOut) outs[wave][t * 32 + c][m] = o[v];
         }
+        }   // the two network passes
         __builtin_amdgcn_s_waitcnt(0);
         __builtin_amdgcn_wave_barrier();
         asm volatile("" ::: "memory");
-        // One lane per ray: lanes 0..31 of the wave.
-        const bool valid = lane < 32 && ray0 + lane < n;
+        // One lane per ray: ray ray0 + lane.
+        const bool valid = ray0 + lane < n;
         const long long i = visit_ray(IA, valid ? ray0 + lane : 0);
         float y[kOut];
 #pragma unroll
-        for (int m = 0; m < kOut; ++m) y[m] = outs[wave][lane & 31][m];
+        for (int m = 0; m < kOut; ++m) y[m] = outs[wave][lane][m];
         __builtin_amdgcn_wave_barrier();
         const float mx = fmaxf(y[0], fmaxf(y[1], y[2]));
         const float e0 = __expf(y[0] - mx), e1 = __expf(y[1] - mx), e2 = __expf(y[2] - mx);

@@ -112,7 +112,7 @@ struct InferArgs {
     long long* judge_list;              // judged ray ids (shaded by the surrogate)
     unsigned long long* counts;         // [0] fallbacks, [1] judged
     long long judge_cap;
-    unsigned long long* tile_counter;   // dynamic 128-ray tile claims (zeroed before the launch)
+    unsigned long long* tile_counter;   // dynamic 256-ray tile claims (zeroed before the launch)
 };
 
 hipError_t launch_pack(const float* params, __bf16* pack0, __bf16* packH, __bf16* pack4, __bf16* WT,
