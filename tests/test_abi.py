@@ -89,3 +89,25 @@ def test_tonemap_host_only():
     assert np.all(np.diff(lum_o[ok][order]) >= -1e-6)
     black = np.zeros((4, 4, 4), np.float32)
     assert np.array_equal(grrt_amd.tonemap(black), black)
+
+
+def test_tonemap_closed_forms():
+    """Cases the curve fixes exactly: a uniform grey frame has its white
+    point at its own luminance, so every channel maps to 1.0; scaling a frame
+    by k scales the log-average exposure by 1/k, so the output is unchanged
+    while k L stays far above the 1e-6 offset of the log average."""
+    import numpy as np
+
+    import grrt_amd
+    grey = np.zeros((5, 7, 4), np.float32)
+    grey[..., :3] = 0.37
+    grey[..., 3] = 1.0
+    out = grrt_amd.tonemap(grey)
+    np.testing.assert_allclose(out[..., :3], 1.0, rtol=0, atol=2e-7)
+    assert np.array_equal(out[..., 3], grey[..., 3])
+    rng = np.random.default_rng(5)
+    fb = np.zeros((16, 16, 4), np.float32)
+    fb[..., :3] = rng.lognormal(0.0, 1.0, (16, 16, 3)).astype(np.float32)
+    a = grrt_amd.tonemap(fb)
+    b = grrt_amd.tonemap(fb * np.float32(64.0))
+    np.testing.assert_allclose(a[..., :3], b[..., :3], rtol=0, atol=2e-6)
