@@ -181,6 +181,7 @@ API = [
                                             C.POINTER(GRRTSurrogateStats)]),
     ("grrt_ext_tonemap_device", C.c_int, [_P, _P, C.POINTER(C.c_double)]),
     ("grrt_ext_get_frame", C.c_int, [_P, _P]),
+    ("grrt_ext_set_integrator", C.c_int, [_P, C.c_int]),
     ("grrt_ext_render_spectral_f32", C.c_int, [_P, C.POINTER(C.c_float), C.c_int, C.c_int]),
     ("grrt_ext_vol_build", _P, [C.POINTER(GRRTParams)]),
     ("grrt_ext_vol_scalars", C.c_int, [_P, C.POINTER(C.c_double), C.c_int]),
@@ -523,6 +524,10 @@ class Context:
         """0 / False: all FP64; 1 / True: FP32 transfer; 2: FP32 transfer and
         FP32 far-field geodesic steps (volumetric path)."""
         self._check(self._L.grrt_ext_set_precision(self._h, int(mode)), "set_precision")
+
+    def set_integrator(self, mode: int):
+        """0: adaptive Dormand-Prince (default); 1: step-doubling RK4."""
+        self._check(self._L.grrt_ext_set_integrator(self._h, int(mode)), "set_integrator")
 
     def render_spectral_f32(self, width=None, height=None) -> np.ndarray:
         w = self.params.width if width is None else width

@@ -239,6 +239,7 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
     P.cancel = ctx->cancel_flag_dev;
     P.refill_min = std::clamp(ctx->tune_refill, 1, 64);
     P.chunk = std::max(ctx->tune_chunk, 64);
+    P.integrator = ctx->integrator;
     if (vol) {
         P.vol = ctx->vol;
         P.nbins = nb;
@@ -272,7 +273,8 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
     check(hipEventRecord(ctx->ev[0], ctx->stream), "hipEventRecord");
     if (vol) check(launch_trace_vol(ctx->params_dev.ptr, n_rays, nb, spectral, ctx->mixed_precision, ctx->stream),
                    "volumetric trace launch");
-    else check(launch_trace_thin(ctx->params_dev.ptr, 0, blocks, threads, ctx->stream), "trace kernel launch");
+    else check(launch_trace_thin(ctx->params_dev.ptr, 0, blocks, threads, ctx->stream, ctx->integrator),
+               "trace kernel launch");
     check(hipEventRecord(ctx->ev[1], ctx->stream), "hipEventRecord");
     if (spectral)
         check(launch_finalize_spectral(P.sample_spec, d_out, npix, nS, nb, kind == Output::kSpectralF32, ctx->stream),
@@ -779,6 +781,15 @@ int grrt_ext_set_precision(GRRTContext* ctx, int mode) {
     });
 }
 
+int grrt_ext_set_integrator(GRRTContext* ctx, int mode) {
+    return guarded(ctx, [&] {
+        if (mode < 0 || mode > 1) throw HipFailure("integrator must be 0 (Dormand-Prince) or 1 (step-doubling RK4)");
+        ctx->integrator = mode;
+        ctx->batch_views.clear();      // the recorded batch carries its parameter block
+        return 0;
+    });
+}
+
 int grrt_ext_render_spectral_f32(GRRTContext* ctx, float* out, int width, int height) {
     return guarded(ctx, [&] { return render_spectral_host(ctx, out, width, height, true, nullptr, nullptr); });
 }
@@ -937,6 +948,7 @@ void prepare_batch(GRRTContext* ctx, const GRRTView* views, int n, uint64_t seed
     P.cancel = nullptr;
     P.refill_min = std::clamp(ctx->tune_refill, 1, 64);
     P.chunk = std::max(ctx->tune_chunk, 64);
+    P.integrator = ctx->integrator;
     check(hipMemcpy(ctx->batch_tp.ptr, &P, sizeof(TraceParams), hipMemcpyHostToDevice), "hipMemcpy(batch params)");
     // Record: reset, the persistent trace over all views, finalize.
     if (ctx->batch_exec) (void)hipGraphExecDestroy(ctx->batch_exec);
@@ -948,7 +960,7 @@ void prepare_batch(GRRTContext* ctx, const GRRTView* views, int n, uint64_t seed
     hipError_t e = launch_reset(P.counters, P.sample_cls, rays, ctx->stream);
     if (e == hipSuccess)
         e = launch_trace_thin(ctx->batch_tp.ptr, 1, std::max(1, ctx->num_cus * ctx->tune_blocks_per_cu),
-                              ctx->tune_block, ctx->stream);
+                              ctx->tune_block, ctx->stream, ctx->integrator);
     if (e == hipSuccess) e = launch_finalize(P.sample_rgb, P.sample_cls, ctx->batch_out.ptr, pix, nS, ctx->stream);
     hipGraph_t g = nullptr;
     const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);

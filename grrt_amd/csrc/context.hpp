@@ -47,7 +47,9 @@ hipError_t launch_trace_vol(const TraceParams* P, long long n_rays, int nbins, b
 hipError_t launch_finalize_spectral(const double* spec, void* out, long long npix, int nS, int nb, bool f32,
                                     hipStream_t st);
 // mode: 0 one view, 1 batch of views, 2 listed rays of one view.
-hipError_t launch_trace_thin(const TraceParams* P, int mode, int blocks, int threads, hipStream_t st);
+// integrator: 0 adaptive Dormand-Prince, 1 step-doubling RK4.
+hipError_t launch_trace_thin(const TraceParams* P, int mode, int blocks, int threads, hipStream_t st,
+                             int integrator);
 hipError_t launch_finalize(const double* rgb, const uint8_t* cls, float* out, long long npix, int nS,
                            hipStream_t st);
 double flops_per_attempt();
@@ -89,6 +91,7 @@ struct GRRTContext {
     grrt_amd::DevBuf<unsigned char> spec_out;
     bool mixed_precision = false;        // FP32 radiative transfer in the volumetric path
     bool far_field_f32 = false;          // and FP32 geodesic steps outside r = 15 M
+    int integrator = 0;                  // 0: adaptive Dormand-Prince, 1: step-doubling RK4
 
     // Per-render buffers.
     grrt_amd::DevBuf<double> sample_rgb;

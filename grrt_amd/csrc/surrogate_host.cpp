@@ -492,6 +492,7 @@ int grrt_ext_render_surrogate(GRRTContext* ctx, float* out, int device_out, doub
         // after the network pass against 3.6 ms with the plain frame's
         // 128-ray claims).
         P.chunk = 64;
+        P.integrator = ctx->integrator;
         std::vector<TraceParams> bp(nb, P);
         for (int k = 0; k < nb; ++k) {
             bp[k].counters = ctx->counters.ptr + k;
@@ -537,7 +538,8 @@ int grrt_ext_render_surrogate(GRRTContext* ctx, float* out, int device_out, doub
             check(launch_mlp_infer(IA, ctx->num_cus, st), "surrogate pass");
             check(hipEventRecord(ctx->band_ev[k], st), "event");
             check(hipStreamWaitEvent(s2, ctx->band_ev[k], 0), "stream wait");
-            check(launch_trace_thin(ctx->params_dev.ptr + k, 2, blocks, ctx->tune_block, s2), "fallback trace");
+            check(launch_trace_thin(ctx->params_dev.ptr + k, 2, blocks, ctx->tune_block, s2, ctx->integrator),
+                  "fallback trace");
             check(launch_judge(S.judge_list.ptr + bands[k].joff, S.counts.ptr + 4 * k, bands[k].cap,
                                ctx->sample_rgb.ptr, ctx->sample_cls.ptr, n_rays + bands[k].joff,
                                S.judge_err.ptr + bands[k].joff, S.counts.ptr + 4 * k + 2, s2),

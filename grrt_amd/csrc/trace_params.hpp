@@ -126,6 +126,7 @@ struct TraceParams {
     VolDev vol;
     int nbins;                          // 3 for RGB, else spectral bins
     int far_f32;                        // FP32 transfer path: FP32 geodesic steps outside r = 15 M
+    int integrator;                     // 0: adaptive Dormand-Prince (the reference tracer's); 1: step-doubling RK4
     const double* nu_obs;               // observer-frame frequencies [Hz]
     const double* lognu_obs;            // log10 of the same
     void* rt_scratch;                   // spectral accumulators, 4 * nbins * n_rays of RT (bins > 64 * kVolBinsPerLane)

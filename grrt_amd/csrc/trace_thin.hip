@@ -72,6 +72,26 @@ __device__ __forceinline__ double hamiltonian_from_rhs(const Ray& R, double dt) 
     return 0.5 * (R.P0 * dt + R.y.v[3] * R.k1.v[0] + R.y.v[4] * R.k1.v[1] + R.P3 * R.k1.v[2]);
 }
 
+// Classic RK4 step of length h from y whose right-hand side k1 is known
+// (include/grrt/geodesic/rk4.h:34-52).
+__device__ __forceinline__ Vec5 rk4_from(const KerrK& K, double P0, double P3, const Vec5& y, const Vec5& k1,
+                                         double h) {
+    Vec5 s, o;
+    double dt;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) s.v[i] = y.v[i] + k1.v[i] * (h * 0.5);
+    const Vec5 k2 = kerr_rhs(K, P0, P3, s, dt);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) s.v[i] = y.v[i] + k2.v[i] * (h * 0.5);
+    const Vec5 k3 = kerr_rhs(K, P0, P3, s, dt);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) s.v[i] = y.v[i] + k3.v[i] * h;
+    const Vec5 k4 = kerr_rhs(K, P0, P3, s, dt);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) o.v[i] = y.v[i] + (k1.v[i] + k2.v[i] * 2.0 + k3.v[i] * 2.0 + k4.v[i]) * (h / 6.0);
+    return o;
+}
+
 // Ray start: camera covector (src/camera.cpp:69-96) and the first
 // right-hand side.
 __device__ __forceinline__ void init_ray(const ViewParams& V, const SampleId& d, const KerrK& K, Ray& R) {
@@ -183,7 +203,10 @@ __device__ __forceinline__ int find_view(const ViewParams* views, int n, unsigne
 //   kMode 2 (list): one view, the queue is the listed ray ids (the surrogate's
 //     fallback rays, then its judged rays, whose results go after the frame's
 //     samples); the list lengths are read from device memory.
-template <int kMode>
+//   kDoubling: the step-doubling RK4 controller (src/rk4.cpp:33-80) in place
+//     of Dormand-Prince; a separate instantiation, so the Dormand-Prince
+//     kernels carry none of its code.
+template <int kMode, bool kDoubling>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void trace_thin(const TraceParams* Pg) {
     constexpr bool kBatch = kMode == 1;
@@ -192,6 +215,7 @@ void trace_thin(const TraceParams* Pg) {
     __shared__ double s_spec[3 * kSpecN];
     const double r_escape = Pg->r_escape, tol = Pg->tol;
     const int max_steps = Pg->max_steps, refill_min = Pg->refill_min, chunk = Pg->chunk;
+    constexpr bool rk4_doubling = kDoubling;
     unsigned long long n_fb = 0;
     long long n_queue = Pg->n_rays;
     if (kList) {
@@ -311,15 +335,29 @@ void trace_thin(const TraceParams* Pg) {
         }
         if (!live) continue;
 
-        // ---- one Dormand–Prince attempt (include/grrt/geodesic/rk4.h:73-177) ----
         const double h = R.dl;
+        Vec5 y5, k7;
+        double dt7 = 0.0, err = 0.0;
+        if (rk4_doubling) {
+            // ---- one step-doubling RK4 attempt (src/rk4.cpp:33-80): a step of
+            // h against two of h/2, error over r, theta, phi, p_r, p_theta
+            // (p_t and p_phi are exact in both) ----
+            const Vec5 full = rk4_from(K, R.P0, R.P3, R.y, R.k1, h);
+            const Vec5 mid = rk4_from(K, R.P0, R.P3, R.y, R.k1, h * 0.5);
+            double dtm;
+            const Vec5 km = kerr_rhs(K, R.P0, R.P3, mid, dtm);
+            y5 = rk4_from(K, R.P0, R.P3, mid, km, h * 0.5);
+#pragma unroll
+            for (int i = 0; i < 5; ++i)
+                err = __builtin_fmax(err, __builtin_fabs(full.v[i] - y5.v[i]) * rcp1(__builtin_fabs(y5.v[i]) + 1e-10));
+        } else {
+        // ---- one Dormand–Prince attempt (include/grrt/geodesic/rk4.h:73-177) ----
         const double h21 = h * A21;
         const double h31 = h * A31, h32 = h * A32;
         const double h41 = h * A41, h42 = h * A42, h43 = h * A43;
         const double h51 = h * A51, h52 = h * A52, h53 = h * A53, h54 = h * A54;
         const double h61 = h * A61, h62 = h * A62, h63 = h * A63, h64 = h * A64, h65 = h * A65;
-        Vec5 s, k2, k3, k4, k5, k6, k7, y5, e;
-        double dt7;
+        Vec5 s, k2, k3, k4, k5, k6, e;
 #pragma unroll
         for (int i = 0; i < 5; ++i) s.v[i] = __builtin_fma(R.k1.v[i], h21, R.y.v[i]);
         k2 = kerr_rhs(K, R.P0, R.P3, s, dt7);
@@ -354,11 +392,11 @@ void trace_thin(const TraceParams* Pg) {
                          __builtin_fma(k4.v[i], e4, __builtin_fma(k3.v[i], e3, R.k1.v[i] * e1))));
         }
         k7 = kerr_rhs(K, R.P0, R.P3, y5, dt7);
-        double err = 0.0;
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
             const double ei = __builtin_fma(k7.v[i], e7, e.v[i]);
             err = __builtin_fmax(err, __builtin_fabs(ei) * rcp1(__builtin_fabs(y5.v[i]) + 1e-10));
+        }
         }
         ++R.attempts;
 
@@ -372,6 +410,18 @@ void trace_thin(const TraceParams* Pg) {
         if (R.retry >= 20) {
             accept = true;           // retries exhausted: take the last attempt
             next = h;
+        } else if (rk4_doubling) {
+            // src/rk4.cpp:60-75: keep h (double it below 1% of tol), or halve
+            if (err <= tol) {
+                accept = true;
+                next = smin(err < tol * 0.01 ? h * 2.0 : h, 5.0 * smax(__builtin_fabs(y5.v[0]), 1.0));
+            } else if (h * 0.5 < 1e-6) {
+                accept = true;       // step too small: keep the two half steps
+                next = smin(1e-6, 5.0 * smax(__builtin_fabs(y5.v[0]), 1.0));
+            } else {
+                accept = false;
+                next = h * 0.5;
+            }
         } else if (err <= tol) {
             accept = true;
             double f = (err > 1e-30) ? step_factor(tol / err) : 5.0;
@@ -395,6 +445,7 @@ void trace_thin(const TraceParams* Pg) {
             continue;
         }
         R.retry = 0;
+        if (rk4_doubling) k7 = kerr_rhs(K, R.P0, R.P3, y5, dt7);   // right-hand side at the new state
 
         // ---- thin-disk crossing between the previous and new state ----
         if (use_disk) {
@@ -485,10 +536,18 @@ hipError_t launch_reset(DevCounters* c, uint8_t* cls, long long n_rays, hipStrea
     return hipGetLastError();
 }
 
-hipError_t launch_trace_thin(const TraceParams* P, int mode, int blocks, int threads, hipStream_t st) {
-    if (mode == 1) hipLaunchKernelGGL(trace_thin<1>, dim3(blocks), dim3(threads), 0, st, P);
-    else if (mode == 2) hipLaunchKernelGGL(trace_thin<2>, dim3(blocks), dim3(threads), 0, st, P);
-    else hipLaunchKernelGGL(trace_thin<0>, dim3(blocks), dim3(threads), 0, st, P);
+hipError_t launch_trace_thin(const TraceParams* P, int mode, int blocks, int threads, hipStream_t st,
+                             int integrator) {
+    const dim3 g(static_cast<unsigned>(blocks)), b(static_cast<unsigned>(threads));
+    if (integrator == 1) {
+        if (mode == 1) hipLaunchKernelGGL((trace_thin<1, true>), g, b, 0, st, P);
+        else if (mode == 2) hipLaunchKernelGGL((trace_thin<2, true>), g, b, 0, st, P);
+        else hipLaunchKernelGGL((trace_thin<0, true>), g, b, 0, st, P);
+    } else {
+        if (mode == 1) hipLaunchKernelGGL((trace_thin<1, false>), g, b, 0, st, P);
+        else if (mode == 2) hipLaunchKernelGGL((trace_thin<2, false>), g, b, 0, st, P);
+        else hipLaunchKernelGGL((trace_thin<0, false>), g, b, 0, st, P);
+    }
     return hipGetLastError();
 }
 

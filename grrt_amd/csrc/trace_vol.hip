@@ -107,6 +107,39 @@ __device__ double adaptive_dp45(const KerrK& K, GState& s, double dl, double tol
     return dl;   // not reached
 }
 
+// Classic RK4 step (include/grrt/geodesic/rk4.h:34-52).
+__device__ GState rk4_step(const KerrK& K, const GState& s, double h);
+
+// Adaptive step-doubling RK4 (src/rk4.cpp:33-80): a step of dl against two
+// of dl/2, error over r, theta, phi, p_r, p_theta; accept (doubling dl
+// below 1% of tol) or halve, keeping the half steps once dl < 1e-6; after
+// 20 halvings the two half steps unchecked.  Returns the next dl.
+__device__ double adaptive_rk4_doubling(const KerrK& K, GState& s, double dl, double tol, int& attempts) {
+    for (int retry = 0; retry < 20; ++retry) {
+        const GState full = rk4_step(K, s, dl);
+        const GState mid = rk4_step(K, s, dl * 0.5);
+        const GState half = rk4_step(K, mid, dl * 0.5);
+        ++attempts;
+        double err = 0.0;
+        for (int i = 0; i < 5; ++i)
+            err = __builtin_fmax(err, __builtin_fabs(full.y.v[i] - half.y.v[i]) *
+                                          rcp1(__builtin_fabs(half.y.v[i]) + 1e-10));
+        if (err <= tol) {
+            s = half;
+            return smin(err < tol * 0.01 ? dl * 2.0 : dl, 5.0 * smax(__builtin_fabs(half.y.v[0]), 1.0));
+        }
+        dl *= 0.5;
+        if (dl < 1e-6) {
+            s = half;
+            return smin(1e-6, 5.0 * smax(__builtin_fabs(half.y.v[0]), 1.0));
+        }
+    }
+    const GState mid = rk4_step(K, s, dl * 0.5);
+    s = rk4_step(K, mid, dl * 0.5);
+    ++attempts;
+    return dl;
+}
+
 // ---- FP32 far-field step ----
 constexpr double kFarField = 15.0;       // in units of M
 constexpr float kFarTol = 1e-6f;
@@ -491,6 +524,7 @@ __global__ __launch_bounds__(256) void trace_vol(const TraceParams* Pg, int grou
             }
         }
         if (kFarFieldF32<RT> && P.far_f32 && r > kFarField * Vw.M) dl = adaptive_dp45_f32(K, s, dl, attempts);
+        else if (P.integrator == 1) dl = adaptive_rk4_doubling(K, s, dl, P.tol, attempts);
         else dl = adaptive_dp45(K, s, dl, P.tol, attempts);
         ++steps;
         bool opaque = false;

@@ -129,6 +129,13 @@ GRRT_EXPORT int grrt_ext_autotune(GRRTContext* ctx, const GRRTView* views, int n
  * ray-march's step sequence: per-pixel intensities then differ from the
  * all-FP64 result by the ray-march's own discretisation error. */
 GRRT_EXPORT int grrt_ext_set_precision(GRRTContext* ctx, int mode);
+/* Adaptive integrator of the geodesic: 0 (default) the Dormand-Prince 5(4)
+ * controller the reference tracer calls (src/rk4.cpp:87-136); 1 the
+ * reference's step-doubling RK4 controller (src/rk4.cpp:33-80: a step of dl
+ * against two of dl/2, accept and double below 1% of tol, else halve).
+ * Applies to the thin-disk, batch, surrogate-fallback and volumetric
+ * tracers. */
+GRRT_EXPORT int grrt_ext_set_integrator(GRRTContext* ctx, int mode);
 /* grrt_render_spectral into a float32 cube (width*height*nbins). */
 GRRT_EXPORT int grrt_ext_render_spectral_f32(GRRTContext* ctx, float* out, int width, int height);
 

@@ -28,6 +28,7 @@ class OracleScene(C.Structure):
         ("stars", C.c_int),
         ("tol", C.c_double), ("max_steps", C.c_int),
         ("r_escape", C.c_double),
+        ("integrator", C.c_int),
     ]
 
 
@@ -121,6 +122,8 @@ def lib(variant: str = "strict") -> C.CDLL:
     L.oracle_rk4_step.argtypes = [_D, _D, _PD, _D, _PD]
     L.oracle_adaptive_step.argtypes = [_D, _D, _PD, _D, _D, _PD, C.POINTER(C.c_int)]
     L.oracle_adaptive_step.restype = _D
+    L.oracle_adaptive_step_doubling.argtypes = [_D, _D, _PD, _D, _D, _PD, C.POINTER(C.c_int)]
+    L.oracle_adaptive_step_doubling.restype = _D
     L.oracle_pixel_hash.argtypes = [C.c_int] * 4
     L.oracle_pixel_hash.restype = _D
     L.oracle_seeded_jitter.argtypes = [C.c_uint64, C.c_int64, _PD, _PD]
@@ -158,12 +161,12 @@ def _dp(a: np.ndarray):
 
 def scene(width=256, height=256, mass=1.0, spin_a=0.0, r_obs=50.0, theta_obs=1.3962634,
           phi_obs=0.0, fov=1.5707963, disk=True, r_in=None, r_out=20.0, t_peak=1e7,
-          stars=False, tol=1e-8, max_steps=10000, r_escape=1000.0) -> OracleScene:
+          stars=False, tol=1e-8, max_steps=10000, r_escape=1000.0, integrator=0) -> OracleScene:
     if r_in is None:
         r_in = lib().oracle_isco(mass, spin_a)
     return OracleScene(width, height, mass, spin_a, r_obs, theta_obs, phi_obs, fov,
                        1 if disk else 0, r_in, r_out, t_peak, 1 if stars else 0,
-                       tol, max_steps, r_escape)
+                       tol, max_steps, r_escape, integrator)
 
 
 def fast_sincos(x: float, variant="strict"):
@@ -219,6 +222,14 @@ def adaptive_step(M, a, y, dl, tol):
     y = np.ascontiguousarray(y, np.float64)
     out, n = np.zeros(8), C.c_int()
     nxt = lib().oracle_adaptive_step(M, a, _dp(y), dl, tol, _dp(out), C.byref(n))
+    return out, nxt, n.value
+
+
+def adaptive_step_doubling(M, a, y, dl, tol):
+    """src/rk4.cpp:33-80: returns (state, next dl, attempts)."""
+    y = np.ascontiguousarray(y, np.float64)
+    out, n = np.zeros(8), C.c_int()
+    nxt = lib().oracle_adaptive_step_doubling(M, a, _dp(y), dl, tol, _dp(out), C.byref(n))
     return out, nxt, n.value
 
 

@@ -32,6 +32,9 @@ typedef struct {
     double tol;
     int max_steps;
     double r_escape;    /* reference: 1000 */
+    int integrator;     /* 0: adaptive Dormand-Prince (src/rk4.cpp:87-136, the
+                           reference tracer's); 1: step-doubling RK4
+                           (src/rk4.cpp:33-80) */
 } OracleScene;
 
 /* Per-ray record.  term: 0 horizon, 1 escaped, 2 max steps.
@@ -74,6 +77,8 @@ void   oracle_dp45_attempt(double M, double a, const double y[8], double dl,
 void   oracle_rk4_step(double M, double a, const double y[8], double dl, double out[8]);
 double oracle_adaptive_step(double M, double a, const double y[8], double dl, double tol,
                             double out[8], int* attempts);
+double oracle_adaptive_step_doubling(double M, double a, const double y[8], double dl, double tol,
+                                     double out[8], int* attempts);
 double oracle_pixel_hash(int i, int j, int s, int ch);
 void   oracle_seeded_jitter(uint64_t seed, int64_t sample_index, double* jx, double* jy);
 void   oracle_camera_ray(const OracleScene* sc, double px, double py, double state[8]);

@@ -284,6 +284,59 @@ double adaptive_step(double M, double a, const State& y, double dlambda, double 
     return dl;
 }
 
+// src/rk4.cpp:33-80 — step-doubling RK4: one step of dl against two of
+// dl/2, relative max-norm error over r, theta, phi and the four momenta
+// (floor 1e-10); accept and double dl when the error is below 1% of tol;
+// otherwise halve dl, keeping the half-step result once dl < 1e-6; after 20
+// halvings take the two half steps unchecked.  Returns the next dl.
+double adaptive_step_doubling(double M, double a, const State& y, double dlambda, double tol,
+                              State& out, int& attempts) {
+    const double dl_min = 1e-6, eps = 1e-10;
+    double dl = dlambda;
+    for (int retry = 0; retry < 20; ++retry) {
+        State full, mid, half;
+        rk4_step(M, a, y, dl, full);
+        rk4_step(M, a, y, dl * 0.5, mid);
+        rk4_step(M, a, mid, dl * 0.5, half);
+        ++attempts;
+        double err = 0.0;
+        for (int i = 1; i < 4; ++i) {
+            const double diff = std::fabs(full.x[i] - half.x[i]);
+            const double scale = std::fabs(half.x[i]) + eps;
+            err = smax(err, diff / scale);
+        }
+        for (int i = 0; i < 4; ++i) {
+            const double diff = std::fabs(full.p[i] - half.p[i]);
+            const double scale = std::fabs(half.p[i]) + eps;
+            err = smax(err, diff / scale);
+        }
+        if (err <= tol) {
+            double next = dl;
+            if (err < tol * 0.01) next = dl * 2.0;
+            next = smin(next, 5.0 * smax(std::fabs(half.x[1]), 1.0));
+            out = half;
+            return next;
+        }
+        dl *= 0.5;
+        if (dl < dl_min) {
+            out = half;
+            return smin(dl_min, 5.0 * smax(std::fabs(half.x[1]), 1.0));
+        }
+    }
+    State mid;
+    rk4_step(M, a, y, dl * 0.5, mid);
+    rk4_step(M, a, mid, dl * 0.5, out);
+    ++attempts;
+    return dl;
+}
+
+// The outer loops' step: the tracer's Dormand–Prince controller, or the
+// step-doubling one when the scene asks for it.
+double outer_step(const OracleScene& p, const State& y, double dl, State& out, int& attempts) {
+    return p.integrator == 1 ? adaptive_step_doubling(p.mass, p.spin_a, y, dl, p.tol, out, attempts)
+                             : adaptive_step(p.mass, p.spin_a, y, dl, p.tol, out, attempts);
+}
+
 // ------------------------------------------------------------ sampling ---
 // src/renderer.cpp:14-20.
 double pixel_hash(int i, int j, int s, int ch) {
@@ -619,7 +672,7 @@ void trace(const Scene& S, const State& init, OracleRay& o, bool diag) {
         if (r > S.p.r_escape) { term = 1; break; }
         prev = st;
         State nxt;
-        dl = adaptive_step(M, a, st, dl, S.p.tol, nxt, o.attempts);
+        dl = outer_step(S.p, st, dl, nxt, o.attempts);
         st = nxt;
         ++o.steps;
         if (diag) o.h_max = smax(o.h_max, std::fabs(hamiltonian(M, a, st.x, st.p)));
@@ -1004,7 +1057,7 @@ void vol_trace(const Scene& S, const Vol& VD, const State& init, int nbins, cons
             }
         }
         State nxt;
-        dl = adaptive_step(M, a, st, dl, S.p.tol, nxt, o.attempts);
+        dl = outer_step(S.p, st, dl, nxt, o.attempts);
         st = nxt;
         ++o.steps;
         const bool opaque = !spectral && (T[0] < 1e-6 && T[1] < 1e-6 && T[2] < 1e-6);
@@ -1097,6 +1150,17 @@ double oracle_adaptive_step(double M, double a, const double y[8], double dl, do
     for (int i = 0; i < 4; ++i) { s.x[i] = y[i]; s.p[i] = y[4 + i]; }
     int n = 0;
     const double next = adaptive_step(M, a, s, dl, tol, o, n);
+    for (int i = 0; i < 4; ++i) { out[i] = o.x[i]; out[4 + i] = o.p[i]; }
+    if (attempts) *attempts = n;
+    return next;
+}
+
+double oracle_adaptive_step_doubling(double M, double a, const double y[8], double dl, double tol,
+                                     double out[8], int* attempts) {
+    State st, o;
+    for (int i = 0; i < 4; ++i) { st.x[i] = y[i]; st.p[i] = y[4 + i]; }
+    int n = 0;
+    const double next = adaptive_step_doubling(M, a, st, dl, tol, o, n);
     for (int i = 0; i < 4; ++i) { out[i] = o.x[i]; out[4 + i] = o.p[i]; }
     if (attempts) *attempts = n;
     return next;

@@ -161,3 +161,34 @@ def test_edge_cases(grrt, oracle):
     orec = oracle.trace_samples(sc, oracle.JITTER_CENTER, 1, 0, np.arange(256))
     assert np.mean(rec["cls"] == orec["cls"]) >= 0.98
     assert np.array_equal(rec["steps"], orec["steps"]) or np.mean(rec["steps"] == orec["steps"]) > 0.9
+
+
+@pytest.mark.parametrize("metric,spin", [(0, 0.0), (1, 0.998)])
+def test_step_doubling_integrator_parity(grrt, oracle, metric, spin):
+    """The reference's step-doubling RK4 controller (src/rk4.cpp:33-80,
+    grrt_ext_set_integrator(ctx, 1)) against the oracle's restatement on the
+    same rays, with the bars of the Dormand-Prince frames; the volumetric
+    RGB frame too (its own bars)."""
+    print(f"case: step-doubling RK4, metric {metric} a={spin}, 128^2 pixel centres", flush=True)
+    p = grrt.make_params(**{**C1, "width": 128, "height": 128, "metric": metric, "spin": spin})
+    with grrt.Context(p) as ctx:
+        ctx.set_sampling(grrt.SAMPLING_CENTER)
+        ctx.set_integrator(1)
+        ctx.set_records(True)
+        fb = ctx.render()
+        rec = ctx.records()
+    sc = _oracle_scene(oracle, p)
+    sc.integrator = 1
+    ofb, ocls, _ = oracle.render(sc, oracle.JITTER_CENTER, 1, threads=0)
+    orec = oracle.trace_samples(sc, oracle.JITTER_CENTER, 1, 0, np.arange(128 * 128))
+    cls = rec["cls"].astype(np.uint8)
+    agree = np.mean(cls == ocls)
+    assert agree >= 0.999, agree
+    mism = (cls != ocls).reshape(128, 128)
+    assert not np.any(mism & ~class_boundary_mask(ocls.reshape(128, 128)))
+    same = (cls == ocls).reshape(128, 128)
+    mre = median_rel_err(fb[..., :3][same], ofb[..., :3][same])
+    assert mre <= 1e-6, mre
+    # the controller's step sequence: attempts per ray as the oracle's
+    assert np.median(np.abs(rec["attempts"] - orec["attempts"])) == 0
+    assert np.median(rec["attempts"]) > 0

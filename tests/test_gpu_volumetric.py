@@ -246,3 +246,27 @@ def test_volumetric_tile_and_spp_groups(grrt, oracle):
         g_vs_o = _median_rel(cube[..., k], ocube[..., k])
         self_noise = _median_rel(fcube[..., k], ocube[..., k])
         assert g_vs_o <= max(20.0 * self_noise, 2e-5), (k, g_vs_o, self_noise)
+
+
+def test_volumetric_step_doubling_parity(grrt, oracle):
+    """The step-doubling RK4 controller (src/rk4.cpp:33-80) in the
+    volumetric tracer's outer loop against the oracle's, RGB 48x48."""
+    print("case: volumetric RGB a=0.9 48x48 pixel centres, step-doubling RK4", flush=True)
+    p, V, sc = _setup(grrt, oracle, 48, 48)
+    sc.integrator = 1
+    with grrt.Context(p) as ctx:
+        ctx.set_sampling(grrt.SAMPLING_CENTER)
+        ctx.set_integrator(1)
+        ctx.set_records(True)
+        ctx.render()
+        rec = ctx.records()
+    ids = np.arange(48 * 48)
+    orec, _ = oracle.vol_trace_samples(sc, V, oracle.JITTER_CENTER, 1, 0, ids)
+    frec, _ = oracle.vol_trace_samples(sc, V, oracle.JITTER_CENTER, 1, 0, ids, variant="fast")
+    assert np.mean(rec["cls"] == orec["cls"]) >= 0.995
+    same = (rec["cls"] == orec["cls"]) & (frec["cls"] == orec["cls"])
+    for c in range(3):
+        g_vs_o = _median_rel(rec["color"][same, c], orec["color"][same, c])
+        self_noise = _median_rel(frec["color"][same, c], orec["color"][same, c])
+        assert g_vs_o <= max(20.0 * self_noise, 2e-5), (c, g_vs_o, self_noise)
+    assert np.any(rec["cls"] == grrt.CLASS_DISK)

@@ -347,3 +347,52 @@ def test_oracle_self_noise(oracle):
     x, y = a[..., :3][same].astype(np.float64), b[..., :3][same].astype(np.float64)
     sel = (x > 0) & (y > 0)
     assert np.median(np.abs(x[sel] - y[sel]) / y[sel]) < 1e-6
+
+
+# ------------------------------------------------- step-doubling RK4 -------
+def test_step_doubling_controller(oracle):
+    """src/rk4.cpp:33-80: the accepted state is two RK4 half steps; an error
+    below 1% of tol doubles the step (capped at 5 max(r,1)); a tolerance no
+    step can meet halves dl until it falls below 1e-6 and keeps that last
+    half-step result."""
+    sc, y = _photon(oracle)
+    a = sc.spin_a
+    dl = 0.05
+    out, nxt, n = oracle.adaptive_step_doubling(M, a, y, dl, 1e-3)
+    half = oracle.rk4_step(M, a, oracle.rk4_step(M, a, y, dl / 2), dl / 2)
+    assert n == 1 and np.array_equal(out, half)
+    assert nxt == min(2 * dl, 5 * max(abs(half[1]), 1.0))
+    # a tolerance nothing meets (negative): 0.05 / 2^k < 1e-6 first at k = 16
+    out, nxt, n = oracle.adaptive_step_doubling(M, a, y, dl, -1.0)
+    k = 16
+    h = dl / 2 ** (k - 1)
+    half = oracle.rk4_step(M, a, oracle.rk4_step(M, a, y, h / 2), h / 2)
+    assert n == k and np.array_equal(out, half) and nxt == 1e-6
+
+
+def test_step_doubling_trace_agrees_with_dp45(oracle):
+    """Both controllers integrate the same geodesics to tol 1e-8: outcome
+    classes agree away from class boundaries, p_t and p_phi stay exact and
+    the Carter constant drifts little.  (Crossing radii and final angles
+    differ by the reference's linear interpolation between step endpoints
+    and by where each run stops past r = 1000, so they are not compared.)"""
+    kw = dict(width=48, height=48, spin_a=0.998, r_obs=50.0, theta_obs=1.3962634, fov=1.5707963)
+    s0 = oracle.scene(**kw)
+    s1 = oracle.scene(integrator=1, **kw)
+    ids = np.arange(48 * 48)
+    r0 = oracle.trace_samples(s0, oracle.JITTER_CENTER, 1, 0, ids)
+    r1 = oracle.trace_samples(s1, oracle.JITTER_CENTER, 1, 0, ids)
+    assert np.mean(r0["cls"] == r1["cls"]) >= 0.99
+    assert np.array_equal(r0["p_final"][:, 0], r1["p_final"][:, 0])
+    assert np.array_equal(r0["p_final"][:, 3], r1["p_final"][:, 3])
+    assert np.sum(r1["cls"] == 2) > 100
+    a = 0.998
+    init = np.array([oracle.camera_ray(s0, (i % 48) + 0.5, (i // 48) + 0.5) for i in ids])
+
+    def carter(x, p):
+        c2, s2 = np.cos(x[:, 2]) ** 2, np.sin(x[:, 2]) ** 2
+        return p[:, 2] ** 2 + c2 * (p[:, 3] ** 2 / s2 - a * a * p[:, 0] ** 2)
+    q0 = carter(init[:, :4], init[:, 4:])
+    q1 = carter(r1["x_final"], r1["p_final"])
+    assert np.max(np.abs(q1 - q0) / np.maximum(1.0, np.abs(q0))) < 1e-6
+    assert np.mean(r1["attempts"]) > np.mean(r0["attempts"])   # RK4 doubling takes more, smaller steps
