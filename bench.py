@@ -336,7 +336,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(rays are independent; the multi-GPU split is described in DESIGN.md)",
+              f"(rays are independent: each GPU would trace its own interleaved tiles)",
               file=sys.stderr)
         return 2
 

@@ -1,6 +1,6 @@
 """GPU parity of the HIP thin-disk tracer against the CPU oracle.
 
-Bars (north star): per-ray outcome class agrees on >= 99.9% of rays with
+Bars: per-ray outcome class agrees on >= 99.9% of rays with
 mismatches only next to a class boundary (the critical curve or a disk
 edge); linear HDR median relative error <= 1e-6; Hamiltonian-constraint
 drift no worse than the oracle's.
