@@ -41,7 +41,7 @@ SG_ENTROPY, SG_LNG_TOL, SG_JUDGE = 0.1, 0.005, 0.02
 TRACE_HBM_BYTES = (3816 * 2 + 103689) * 1024
 
 
-def c2_params(grrt):
+def kerr_frame_params(grrt):
     return grrt.make_params(width=W, height=H, metric=grrt.METRIC_KERR, spin=0.998,
                             observer_r=50.0, observer_theta=math.radians(80.0),
                             observer_phi=0.0, fov=math.radians(90.0), disk=True,
@@ -99,7 +99,7 @@ def disk_pixels(*classes):
     return m
 
 
-def surrogate_section(grrt, ctx, plain_value, steps, warmup, sg_steps, c5_frames):
+def surrogate_section(grrt, ctx, plain_value, steps, warmup, sg_steps, big_frames):
     """Train the surrogate on rays of the FP64 integrator, then time the
     surrogate + fallback + judge frame on the same workload and compare it
     pixel by pixel with the plain FP64 frame."""
@@ -157,43 +157,43 @@ def surrogate_section(grrt, ctx, plain_value, steps, warmup, sg_steps, c5_frames
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / BF16_PEAK_TFLOPS, 4), "traffic": None},
     }
-    if c5_frames > 0:
+    if big_frames > 0:
         # The surrogate's own inference frame: Kerr a=0.998, theta=80 deg,
         # 2048^2 x 4 spp (no star background named there).
-        p5 = c2_params(grrt)
-        p5.width = p5.height = 2048
-        p5.background_type = 0
-        with grrt.Context(p5) as c5:
-            c5.set_sampling(grrt.SAMPLING_SEEDED, SPP, SEED)
-            c5.surrogate_init(1)
-            c5.surrogate_params(ctx.surrogate_params())
-            plain5 = c5.render()
-            p5cls = c5.classes()
-            c5.render_device()
+        pbig = kerr_frame_params(grrt)
+        pbig.width = pbig.height = 2048
+        pbig.background_type = 0
+        with grrt.Context(pbig) as cbig:
+            cbig.set_sampling(grrt.SAMPLING_SEEDED, SPP, SEED)
+            cbig.surrogate_init(1)
+            cbig.surrogate_params(ctx.surrogate_params())
+            plain_big = cbig.render()
+            plain_big_cls = cbig.classes()
+            cbig.render_device()
             t0 = time.perf_counter()
-            for _ in range(c5_frames):
-                c5.render_device()
-            plain_ms = (time.perf_counter() - t0) / c5_frames * 1e3
-            f5, _ = c5.render_surrogate(SG_ENTROPY, SG_JUDGE, SG_LNG_TOL)
-            s5cls = c5.classes()
-            med5, within5 = pixel_flux_error(f5, plain5, disk_pixels(p5cls, s5cls))
-            c5.render_surrogate(SG_ENTROPY, SG_JUDGE, SG_LNG_TOL, device=True)
+            for _ in range(big_frames):
+                cbig.render_device()
+            plain_ms = (time.perf_counter() - t0) / big_frames * 1e3
+            f_big, _ = cbig.render_surrogate(SG_ENTROPY, SG_JUDGE, SG_LNG_TOL)
+            sg_big_cls = cbig.classes()
+            med_big, within_big = pixel_flux_error(f_big, plain_big, disk_pixels(plain_big_cls, sg_big_cls))
+            cbig.render_surrogate(SG_ENTROPY, SG_JUDGE, SG_LNG_TOL, device=True)
             t0 = time.perf_counter()
-            for _ in range(c5_frames):
-                _, st5 = c5.render_surrogate(SG_ENTROPY, SG_JUDGE, SG_LNG_TOL, device=True)
-            sg_ms = (time.perf_counter() - t0) / c5_frames * 1e3
-        r5 = 2048 * 2048 * SPP
-        out["c5_2048sq_4spp"] = {
-            "plain_mrays_s": round(r5 / plain_ms / 1e3, 3), "surrogate_mrays_s": round(r5 / sg_ms / 1e3, 3),
+            for _ in range(big_frames):
+                _, st_big = cbig.render_surrogate(SG_ENTROPY, SG_JUDGE, SG_LNG_TOL, device=True)
+            sg_ms = (time.perf_counter() - t0) / big_frames * 1e3
+        rays_big = 2048 * 2048 * SPP
+        out["frame_2048sq_4spp"] = {
+            "plain_mrays_s": round(rays_big / plain_ms / 1e3, 3), "surrogate_mrays_s": round(rays_big / sg_ms / 1e3, 3),
             "speedup": round(plain_ms / sg_ms, 3), "plain_ms": round(plain_ms, 3), "surrogate_ms": round(sg_ms, 3),
-            "class_agreement_vs_plain": round(float(np.mean(s5cls == p5cls)), 6),
-            "median_disk_pixel_flux_rel_err_vs_plain": round(med5, 5),
-            "fallback_frac": round(st5["fallback_rays"] / st5["rays"], 5),
-            "judge_median_rel_err": round(st5["judge_median_rel_err"], 5)}
+            "class_agreement_vs_plain": round(float(np.mean(sg_big_cls == plain_big_cls)), 6),
+            "median_disk_pixel_flux_rel_err_vs_plain": round(med_big, 5),
+            "fallback_frac": round(st_big["fallback_rays"] / st_big["rays"], 5),
+            "judge_median_rel_err": round(st_big["judge_median_rel_err"], 5)}
     return out
 
 
-def c4_views(seed: int = 7, n: int = 64):
+def uneven_views(seed: int = 7, n: int = 64):
     """The batched-views workload: n thin-disk Kerr views drawn from one
     PRNG stream in a fixed order per view (a, theta, phi, log10 r, fov,
     resolution): a ~ U[0, 0.998], theta ~ U[5, 89] deg, phi ~ U[0, 360) deg,
@@ -213,10 +213,10 @@ def c4_views(seed: int = 7, n: int = 64):
     return out
 
 
-def c4_section(grrt, reps: int) -> dict:
+def batched_views_section(grrt, reps: int) -> dict:
     """64 uneven views in one persistent launch (recorded graph, autotuned
     launch shape) against the same views rendered one after another."""
-    views = c4_views()
+    views = uneven_views()
     rays = sum(v[5] * v[6] for v in views)
     base = grrt.make_params(width=64, height=64, metric=grrt.METRIC_KERR, spin=0.5, disk=True, disk_outer=20.0,
                             stars=False, tolerance=1e-8, max_steps=10000)
@@ -255,27 +255,27 @@ def c4_section(grrt, reps: int) -> dict:
             "batch_speedup": round(seq_ms / batch_ms, 3)}
 
 
-C3_FREQS = [10 ** (14 + 4 * k / 31) for k in range(32)]
+CUBE_FREQS = [10 ** (14 + 4 * k / 31) for k in range(32)]
 
 
-def c3_params(grrt):
+def spectral_cube_params(grrt):
     return grrt.make_params(width=512, height=512, metric=grrt.METRIC_KERR, spin=0.9,
                             observer_theta=math.radians(75.0), fov=math.radians(90.0), volumetric=True,
                             disk_outer=20.0, turbulence=0.4, seed=42, octaves=2, alpha=0.1,
-                            freqs=C3_FREQS)
+                            freqs=CUBE_FREQS)
 
 
-def c3_section(grrt, reps: int, cpu: bool) -> dict:
+def spectral_cube_section(grrt, reps: int, cpu: bool) -> dict:
     """The volumetric spectral cube: Kerr a=0.9, 512^2 x 1 spp (pixel-hash
     jitter), theta=75 deg, alpha=0.1, turbulence 0.4, 2 octaves, seed 42,
     32 frequencies log-spaced over 1e14-1e18 Hz, float32 cube; timed in the
     mixed form (FP32 transfer, FP64 geodesic) and all-FP64, with the CPU
     oracle (all FP64) on a random sample of the same rays."""
-    p = c3_params(grrt)
+    p = spectral_cube_params(grrt)
     t0 = time.perf_counter()
     ctx = grrt.Context(p)
     setup_s = time.perf_counter() - t0
-    ctx.set_frequency_bins(C3_FREQS)
+    ctx.set_frequency_bins(CUBE_FREQS)
     rays = 512 * 512
     out = {"workload": "Kerr a=0.9 512^2 x 1 spp, theta=75deg, volumetric alpha=0.1 turb 0.4 2 oct seed 42, "
                        "32 bins 1e14-1e18 Hz, float32 cube",
@@ -305,12 +305,12 @@ def c3_section(grrt, reps: int, cpu: bool) -> dict:
         rng = np.random.default_rng(2025)
         probe = np.sort(rng.choice(rays, 1000, replace=False))
         t1 = time.perf_counter()
-        O.vol_trace_samples(sc, V, O.JITTER_HASH, 1, 0, probe, freqs=C3_FREQS, threads=threads, variant="fast")
+        O.vol_trace_samples(sc, V, O.JITTER_HASH, 1, 0, probe, freqs=CUBE_FREQS, threads=threads, variant="fast")
         rate = len(probe) / (time.perf_counter() - t1)
         n = int(min(rays, max(1000, rate * 10.0)))
         ids = np.sort(rng.choice(rays, n, replace=False))
         t1 = time.perf_counter()
-        O.vol_trace_samples(sc, V, O.JITTER_HASH, 1, 0, ids, freqs=C3_FREQS, threads=threads, variant="fast")
+        O.vol_trace_samples(sc, V, O.JITTER_HASH, 1, 0, ids, freqs=CUBE_FREQS, threads=threads, variant="fast")
         dt = time.perf_counter() - t1
         cube_s = rays / (n / dt)
         out["cpu_baseline"] = {"value": round(n / dt / 1e6, 5), "unit": "Mrays/s", "cores": threads,
@@ -330,9 +330,9 @@ def main() -> int:
     ap.add_argument("--tune", default="", help="block,blocks_per_cu,refill_min,chunk[,tile]")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--sg-steps", type=int, default=60000, help="surrogate training steps")
-    ap.add_argument("--c5-frames", type=int, default=2, help="2048^2 frames timed per form (0: skip)")
-    ap.add_argument("--c4-reps", type=int, default=2, help="replays of the 64-view batch (0: skip)")
-    ap.add_argument("--c3-reps", type=int, default=3, help="volumetric spectral cubes timed per form (0: skip)")
+    ap.add_argument("--big-frames", type=int, default=2, help="2048^2 frames timed per form (0: skip)")
+    ap.add_argument("--batch-reps", type=int, default=2, help="replays of the 64-view batch (0: skip)")
+    ap.add_argument("--cube-reps", type=int, default=3, help="volumetric spectral cubes timed per form (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
@@ -342,7 +342,7 @@ def main() -> int:
 
     import grrt_amd as grrt
 
-    params = c2_params(grrt)
+    params = kerr_frame_params(grrt)
     ctx = grrt.Context(params)
     ctx.set_sampling(grrt.SAMPLING_SEEDED, SPP, SEED)
     if args.tune:
@@ -410,12 +410,12 @@ def main() -> int:
     }
     if not args.no_surrogate:
         out["surrogate"] = surrogate_section(grrt, ctx, value, args.steps, args.warmup, args.sg_steps,
-                                             args.c5_frames)
+                                             args.big_frames)
     ctx.close()
-    if args.c4_reps > 0:
-        out["batched_views"] = c4_section(grrt, args.c4_reps)
-    if args.c3_reps > 0:
-        out["volumetric_spectral"] = c3_section(grrt, args.c3_reps, not args.no_cpu_baseline)
+    if args.batch_reps > 0:
+        out["batched_views"] = batched_views_section(grrt, args.batch_reps)
+    if args.cube_reps > 0:
+        out["volumetric_spectral"] = spectral_cube_section(grrt, args.cube_reps, not args.no_cpu_baseline)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -14,7 +14,7 @@ from parity_util import class_boundary_mask, median_rel_err
 
 pytestmark = pytest.mark.gpu
 
-C1 = dict(width=256, height=256, metric=0, spin=0.0, observer_r=50.0,
+SCHW_256 = dict(width=256, height=256, metric=0, spin=0.0, observer_r=50.0,
           observer_theta=math.radians(80.0), fov=math.radians(90.0), disk=True,
           disk_outer=20.0, stars=False)
 
@@ -30,9 +30,9 @@ def _oracle_scene(O, p, stars=None):
                    tol=p.integrator_tolerance, max_steps=p.integrator_max_steps)
 
 
-def test_c1_frame_parity(grrt, oracle):
-    print("case: C1 Schwarzschild 256^2 pixel centres", flush=True)
-    p = grrt.make_params(**C1)
+def test_schwarzschild_256_frame_parity(grrt, oracle):
+    print("case: Schwarzschild 256^2 pixel centres", flush=True)
+    p = grrt.make_params(**SCHW_256)
     with grrt.Context(p) as ctx:
         ctx.set_sampling(grrt.SAMPLING_CENTER)
         ctx.set_records(True)
@@ -65,7 +65,7 @@ def test_c1_frame_parity(grrt, oracle):
 def test_shadow_closed_form(grrt):
     """Disk off, a=0: captured iff the angle to the optical axis is below the
     static-observer shadow angle sin(psi) = 3 sqrt(3) M sqrt(1-2M/r)/r."""
-    p = grrt.make_params(**{**C1, "disk": False})
+    p = grrt.make_params(**{**SCHW_256, "disk": False})
     with grrt.Context(p) as ctx:
         ctx.set_sampling(grrt.SAMPLING_CENTER)
         ctx.render()
@@ -106,10 +106,10 @@ def test_hash_sampling_abi_parity(grrt, oracle):
     assert mre <= 1e-6, mre
 
 
-def test_c2_sampled_parity(grrt, oracle):
+def test_kerr_1024_4spp_sampled_parity(grrt, oracle):
     """The 1024^2 4-spp Kerr a=0.998 frame with seeded jitter (seed 1234),
     compared ray by ray with the oracle on 4096 sampled rays."""
-    print("case: C2 Kerr a=0.998 1024^2 4spp seeded 1234 stars", flush=True)
+    print("case: Kerr a=0.998 1024^2 4spp seeded 1234 stars", flush=True)
     p = grrt.make_params(width=1024, height=1024, spin=0.998, observer_theta=math.radians(80),
                          fov=math.radians(90), stars=True, spp=4)
     with grrt.Context(p) as ctx:
@@ -170,7 +170,7 @@ def test_step_doubling_integrator_parity(grrt, oracle, metric, spin):
     same rays, with the bars of the Dormand-Prince frames; the volumetric
     RGB frame too (its own bars)."""
     print(f"case: step-doubling RK4, metric {metric} a={spin}, 128^2 pixel centres", flush=True)
-    p = grrt.make_params(**{**C1, "width": 128, "height": 128, "metric": metric, "spin": spin})
+    p = grrt.make_params(**{**SCHW_256, "width": 128, "height": 128, "metric": metric, "spin": spin})
     with grrt.Context(p) as ctx:
         ctx.set_sampling(grrt.SAMPLING_CENTER)
         ctx.set_integrator(1)

@@ -111,11 +111,11 @@ def test_volumetric_mixed_precision_error(grrt, oracle):
     assert max(errs) <= 1e-3
 
 
-def test_c3_spectral_cube_full_size(grrt, oracle):
+def test_spectral_cube_512sq_32bins_full_size(grrt, oracle):
     """512x512x32 float32 cube (Kerr a=0.9, theta=75 deg) in the mixed
     form: size-independent properties over the whole cube, ray-by-ray
     agreement with the FP64 oracle on sampled pixels."""
-    print("case: C3 512^2 x 32 mixed precision", flush=True)
+    print("case: spectral cube 512^2 x 32 bins, mixed precision", flush=True)
     W = 512
     p, V, sc = _setup(grrt, oracle, W, W, freqs=FREQS)
     with grrt.Context(p) as ctx:
@@ -132,17 +132,17 @@ def test_c3_spectral_cube_full_size(grrt, oracle):
     _, ospec = oracle.vol_trace_samples(sc, V, oracle.JITTER_HASH, 1, 0, pix, freqs=FREQS, threads=0)
     got = cube.reshape(-1, 32)[pix]
     errs = [_median_rel(got[:, k], ospec[:, k]) for k in range(32)]
-    print("C3 per-frequency median rel err:", " ".join(f"{e:.1e}" for e in errs),
+    print("cube per-frequency median rel err:", " ".join(f"{e:.1e}" for e in errs),
           f"trace {st.trace_ms:.1f} ms", flush=True)
     assert max(errs) <= 1e-3
 
 
 def test_far_field_fp32_error_stated(grrt, oracle):
-    """Precision mode 2 (FP32 transfer and FP32 far-field geodesic, the C3
-    policy): the disk entry moves by ~1e-5 M, so each ray's ray-march step
-    sequence changes and per-pixel intensities differ from the all-FP64
-    oracle by the ray-march's discretisation error; the image-summed
-    spectrum is far less sensitive.  Both are measured and bounded."""
+    """Precision mode 2 (FP32 transfer and FP32 far-field geodesic, the
+    spectral cube's precision policy): the disk entry moves by ~1e-5 M, so
+    each ray's ray-march step sequence changes and per-pixel intensities
+    differ from the all-FP64 oracle by the ray-march's discretisation error;
+    the image-summed spectrum is far less sensitive.  Both are measured and bounded."""
     print("case: volumetric spectral 32 bins 48x48, FP32 far field", flush=True)
     p, V, sc = _setup(grrt, oracle, 48, 48, freqs=FREQS)
     with grrt.Context(p) as ctx:
@@ -192,7 +192,7 @@ def test_spectral_fits_stream_equals_cube(grrt, oracle, tmp_path):
 def test_grouped_bins_equal_lane_per_ray(grrt, oracle):
     """The spectral kernel holds up to 256 bins in registers, spread over a
     group of lanes per ray; longer lists run one lane per ray with the bins in
-    device scratch.  A 257-bin list made only of the 32 C3 frequencies, with
+    device scratch.  A 257-bin list made only of the 32 cube frequencies, with
     the step-control bin N/2 on the same frequency, takes the second path and
     must give every copy of each frequency bit for bit the 32-bin grouped
     result (all-FP64 and FP32-transfer modes)."""
