@@ -36,9 +36,10 @@ SURROGATE_FLOPS_PER_RAY = 2.0 * (5 * 256 + 3 * 256 * 256 + 256 * 9)
 # Surrogate gates: class entropy (nats), redshift consistency (ln g), judged share.
 SG_ENTROPY, SG_LNG_TOL, SG_JUDGE = 0.1, 0.005, 0.02
 # HBM bytes moved by one plain-frame trace launch: rocprofv3 --pmc FETCH_SIZE
-# (3 816 KiB, x2 by a copy-kernel calibration) + WRITE_SIZE (103 689 KiB,
-# x1), i.e. the 25-byte per-sample outputs plus table refills.
-TRACE_HBM_BYTES = (3816 * 2 + 103689) * 1024
+# (10 174 KiB, x2 by a copy-kernel calibration) + WRITE_SIZE (118 150 KiB,
+# x1), i.e. the 25-byte per-sample outputs, table refills and the few
+# registers spilled outside the attempt loop.
+TRACE_HBM_BYTES = (10174 * 2 + 118150) * 1024
 
 
 def kerr_frame_params(grrt):
