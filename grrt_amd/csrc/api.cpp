@@ -34,11 +34,6 @@ namespace grrt_api {
 
 thread_local std::string g_last_error;
 
-// The batch tracer carries every lane's own view constants (195 VGPRs), so
-// two waves per SIMD are resident at most: more blocks per CU than two
-// would only wait for the first two to drain the queue.
-constexpr int kBatchMaxBlocksPerCu = 2;
-int batch_blocks_per_cu(const GRRTContext* ctx) { return std::min(ctx->tune_blocks_per_cu, kBatchMaxBlocksPerCu); }
 
 // Volumetric disk parameters exactly as the reference wires them
 // (src/api.cpp:79-127): the disk takes the raw mass and spin fields, its
@@ -964,7 +959,7 @@ void prepare_batch(GRRTContext* ctx, const GRRTView* views, int n, uint64_t seed
     check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
     hipError_t e = launch_reset(P.counters, P.sample_cls, rays, ctx->stream);
     if (e == hipSuccess)
-        e = launch_trace_thin(ctx->batch_tp.ptr, 1, std::max(1, ctx->num_cus * batch_blocks_per_cu(ctx)),
+        e = launch_trace_thin(ctx->batch_tp.ptr, 1, std::max(1, ctx->num_cus * ctx->tune_blocks_per_cu),
                               ctx->tune_block, ctx->stream, ctx->integrator);
     if (e == hipSuccess) e = launch_finalize(P.sample_rgb, P.sample_cls, ctx->batch_out.ptr, pix, nS, ctx->stream);
     hipGraph_t g = nullptr;
@@ -1052,7 +1047,6 @@ int grrt_ext_autotune(GRRTContext* ctx, const GRRTView* views, int n, int best[4
             for (int bpc : bpcs)
                 for (int rf : refills)
                     for (int ch : chunks) {
-                        if (views && bpc > kBatchMaxBlocksPerCu) continue;
                         ctx->tune_tile = tile;
                         ctx->tune_blocks_per_cu = bpc;
                         ctx->tune_refill = rf;

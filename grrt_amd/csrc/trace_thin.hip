@@ -206,12 +206,14 @@ __device__ __forceinline__ int find_view(const ViewParams* views, int n, unsigne
 //   kDoubling: the step-doubling RK4 controller (src/rk4.cpp:33-80) in place
 //     of Dormand-Prince; a separate instantiation, so the Dormand-Prince
 //     kernels carry none of its code.
-// Register budget: the frame kernel is held to 168 VGPRs, three waves per
-// SIMD (a few spilled dwords, none in the attempt loop: 17.1 ms against
-// 17.4 ms at 189 VGPRs and two waves); the batch and list kernels, which
-// carry more per-lane state, keep two waves and no spills.
+// Register budget: the frame and batch kernels are held to 168 VGPRs,
+// three waves per SIMD, with a few dwords spilled outside the attempt
+// block (Kerr a=0.998 1024^2 x 4 frame: 17.1 ms against 17.4 ms at 189
+// VGPRs and two waves; the 64-view batch: 128 ms against 133 ms).  The list
+// kernel, launched one block per CU for the surrogate's fallback rays, keeps
+// two waves and no spills.
 template <int kMode, bool kDoubling>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMode == 0 ? 3 : 2)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(kMode == 2 ? 2 : 3)))
 void trace_thin(const TraceParams* Pg) {
     constexpr bool kBatch = kMode == 1;
     constexpr bool kList = kMode == 2;

@@ -114,7 +114,7 @@ GRRT_EXPORT int grrt_ext_render_batch(GRRTContext* ctx, const GRRTView* views, i
 
 /* Picks the persistent kernel's launch shape by timing candidates on the
  * real workload: the context's frame (views == NULL) or the given batch.
- * Candidates: pixel tile (0 = row order, 8), blocks per CU (1, 2), refill
+ * Candidates: pixel tile (0 = row order, 8), blocks per CU (1, 2, 3), refill
  * threshold (2, 4, 8, 16 idle lanes), queue chunk (64, 128).  The best is
  * kept (for the batch: recorded with its graph) and written to
  * best[4] = {tile, blocks_per_cu, refill_min, chunk}; best_ms its time. */

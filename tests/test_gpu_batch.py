@@ -130,7 +130,7 @@ def test_autotune(grrt):
         ref = ctx.render_batch(VIEWS[:3])
         best, ms = ctx.autotune(VIEWS[:3])
         print("batch best", best, f"{ms:.3f} ms", flush=True)
-        assert best["tile"] in (0, 8) and best["blocks_per_cu"] in (1, 2)
+        assert best["tile"] in (0, 8) and best["blocks_per_cu"] in (1, 2, 3)
         assert best["refill_min"] in (2, 4, 8, 16) and best["chunk"] in (64, 128)
         assert ms > 0
         got = ctx.render_batch(VIEWS[:3])
