@@ -280,7 +280,20 @@ def spectral_cube_section(grrt, reps: int, cpu: bool) -> dict:
     rays = 512 * 512
     out = {"workload": "Kerr a=0.9 512^2 x 1 spp, theta=75deg, volumetric alpha=0.1 turb 0.4 2 oct seed 42, "
                        "32 bins 1e14-1e18 Hz, float32 cube",
-           "context_setup_s": round(setup_s, 2)}
+           "context_setup_s": round(setup_s, 3)}
+    # Set-up of the disk tables: opacity tables on host threads against the
+    # GPU (grrt_create's path), then the radial and vertical structure on
+    # the host in both; best of three builds each.
+    builds = {}
+    for dev in (False, True):
+        runs = [grrt.volumetric_tables(p, device=dev) for _ in range(3)]
+        builds["device" if dev else "host"] = (min(r["build_ms_opacity"] for r in runs),
+                                               min(r["build_ms_total"] for r in runs))
+    out["table_build_ms"] = {"opacity_host": round(builds["host"][0], 2),
+                             "opacity_device": round(builds["device"][0], 2),
+                             "total_host": round(builds["host"][1], 2),
+                             "total_device": round(builds["device"][1], 2),
+                             "host_threads": max(1, min(16, os.cpu_count() or 1))}
     for mode, name in ((1, "mixed"), (0, "fp64")):
         ctx.set_precision(mode)
         ctx.render_spectral_f32()

@@ -184,6 +184,7 @@ API = [
     ("grrt_ext_set_integrator", C.c_int, [_P, C.c_int]),
     ("grrt_ext_render_spectral_f32", C.c_int, [_P, C.POINTER(C.c_float), C.c_int, C.c_int]),
     ("grrt_ext_vol_build", _P, [C.POINTER(GRRTParams)]),
+    ("grrt_ext_vol_build_device", _P, [C.POINTER(GRRTParams)]),
     ("grrt_ext_vol_scalars", C.c_int, [_P, C.POINTER(C.c_double), C.c_int]),
     ("grrt_ext_vol_table", C.c_int64, [_P, C.c_char_p, C.POINTER(C.c_double), C.c_int64]),
     ("grrt_ext_vol_free", None, [_P]),
@@ -193,7 +194,7 @@ API = [
 
 VOL_SCALARS = ["r_isco", "r_horizon", "r_min", "r_outer", "taper_width", "E_isco", "L_isco",
                "rho_scale", "noise_scale", "turbulence", "octaves", "n_r", "n_z", "n_nu",
-               "log_nu_min", "log_nu_max", "M", "a", "t_peak"]
+               "log_nu_min", "log_nu_max", "M", "a", "t_peak", "build_ms_opacity", "build_ms_total"]
 VOL_TABLES = ["H", "z_max", "rho_mid", "T_eff", "rho_profile", "T_profile", "kappa_abs",
               "kappa_es", "kappa_ross", "mu", "perm"]
 
@@ -252,13 +253,16 @@ def make_params(width=256, height=256, metric=METRIC_KERR, spin=0.998, mass=1.0,
     return p
 
 
-def volumetric_tables(params: GRRTParams) -> dict:
-    """Host-side volumetric disk set-up (no GPU): scalars and tables."""
+def volumetric_tables(params: GRRTParams, device: bool = False) -> dict:
+    """Volumetric disk set-up: scalars and tables.  device=False builds on
+    host threads (no GPU needed); device=True computes the opacity tables on
+    the GPU, as grrt_create does."""
     L = lib()
-    h = L.grrt_ext_vol_build(C.byref(params))
+    fn = L.grrt_ext_vol_build_device if device else L.grrt_ext_vol_build
+    h = fn(C.byref(params))
     if not h:
         err = L.grrt_last_error()
-        raise RuntimeError("grrt_ext_vol_build failed: " + (err.decode() if err else "unknown"))
+        raise RuntimeError("volumetric table build failed: " + (err.decode() if err else "unknown"))
     try:
         sc = np.zeros(len(VOL_SCALARS))
         L.grrt_ext_vol_scalars(h, sc.ctypes.data_as(C.POINTER(C.c_double)), len(sc))

@@ -85,7 +85,7 @@ void build_volumetric_tables(GRRTContext* ctx) {
     const auto t0 = std::chrono::steady_clock::now();
     const VolSetup vs = resolve_volumetric(ctx->params);
     VolTables T;
-    build_volumetric(vs.M, vs.a, vs.r_outer, vs.t_peak, vs.vp, T);
+    build_volumetric(vs.M, vs.a, vs.r_outer, vs.t_peak, vs.vp, T, &ctx->stream);
     std::vector<double> lnk(T.op.kappa_abs.size());
     for (size_t i = 0; i < lnk.size(); ++i) lnk[i] = std::log(std::max(T.op.kappa_abs[i], 1e-100));
     upload(ctx->vol_H, T.H, ctx->stream);
@@ -815,13 +815,34 @@ GRRTVolTables* grrt_ext_vol_build(const GRRTParams* params) {
     return nullptr;
 }
 
+GRRTVolTables* grrt_ext_vol_build_device(const GRRTParams* params) {
+    if (!params) { g_last_error = "grrt_ext_vol_build_device: params is null"; return nullptr; }
+    hipStream_t st = nullptr;
+    GRRTVolTables* h = nullptr;
+    try {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n < 1) throw std::runtime_error("no GPU");
+        if (hipStreamCreate(&st) != hipSuccess) throw std::runtime_error("hipStreamCreate failed");
+        const VolSetup vs = resolve_volumetric(*params);
+        h = new GRRTVolTables();
+        grrt_amd::build_volumetric(vs.M, vs.a, vs.r_outer, vs.t_peak, vs.vp, h->t, &st);
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        delete h;
+        h = nullptr;
+    }
+    if (st) (void)hipStreamDestroy(st);
+    return h;
+}
+
 int grrt_ext_vol_scalars(const GRRTVolTables* h, double* out, int cap) {
     if (!h || !out) return -1;
     const auto& t = h->t;
     const double v[] = {t.r_isco, t.r_horizon, t.r_min, t.r_outer, t.taper_width, t.E_isco, t.L_isco,
                         t.rho_scale, t.noise_scale, t.prm.turbulence, static_cast<double>(t.prm.noise_octaves),
                         static_cast<double>(t.n_r), static_cast<double>(t.n_z), static_cast<double>(t.op.n_nu),
-                        t.op.log_nu_min, t.op.log_nu_max, t.M, t.a, t.t_peak};
+                        t.op.log_nu_min, t.op.log_nu_max, t.M, t.a, t.t_peak, t.build_ms_opacity,
+                        t.build_ms_total};
     const int n = static_cast<int>(sizeof(v) / sizeof(v[0]));
     for (int i = 0; i < n && i < cap; ++i) out[i] = v[i];
     return n;

@@ -6,6 +6,8 @@
 #include <cstdint>
 #include <vector>
 
+#include "opacity_math.hpp"
+
 namespace grrt_amd {
 
 struct VolParams {
@@ -19,19 +21,6 @@ struct VolParams {
     double opacity_nu_max = 1e16;
 };
 
-// Ionisation state of the H/He/C/O/Fe mixture (Saha equation).
-struct Ionization {
-    double n_e, mu, n_ion_eff, n_HI, n_Hminus;
-    double n_pop[5][5];
-};
-
-Ionization saha(double rho_cgs, double T);
-double kappa_abs_point(double nu, double rho_cgs, double T, const Ionization& ion);
-double kappa_es_point(double rho_cgs, const Ionization& ion);
-double alpha_ff_point(double nu, double T, const Ionization& ion);
-double alpha_hminus_point(double nu, double T, const Ionization& ion);
-double alpha_bf_point(double nu, double T, const Ionization& ion);
-double planck_nu_host(double nu, double T);
 
 struct OpacityTables {
     int n_nu = 0, n_rho = 100, n_T = 100;
@@ -46,8 +35,16 @@ struct OpacityTables {
     double lookup_mu(double rho, double T) const;
 };
 
-OpacityTables build_opacity_tables(double rho_min, double rho_max, double T_min, double T_max,
-                                   double nu_min, double nu_max);
+// Table geometry (src/opacity.cpp:237-262): 100 x 100 log-spaced (rho, T)
+// cells over the given ranges, max(20, 10 per decade) absorption
+// frequencies; the tables are allocated and zero.
+OpacityTables opacity_table_shape(double rho_min, double rho_max, double T_min, double T_max,
+                                  double nu_min, double nu_max);
+// Every cell of the tables (src/opacity.cpp:263-303), on host threads, or on
+// the GPU (one lane per cell, results copied back; stream-ordered and
+// synchronised before returning).
+void fill_opacity_tables_host(OpacityTables& o);
+hipError_t fill_opacity_tables_device(OpacityTables& o, hipStream_t st);
 
 struct VolTables {
     double M = 1, a = 0, r_outer = 30, t_peak = 1e7;
@@ -59,10 +56,13 @@ struct VolTables {
     std::vector<double> rho_profile, T_profile;     // n_r * n_z
     OpacityTables op;
     int perm[512];
+    double build_ms_opacity = 0.0, build_ms_total = 0.0;   // wall time of the set-up
 };
 
 // Builds everything (src/volumetric_disk.cpp:33-73 construction order).
+// With a stream, the opacity tables are computed on the GPU on that stream;
+// without one, on host threads.  Either way the tables end in host memory.
 void build_volumetric(double M, double a, double r_outer, double t_peak, const VolParams& p,
-                      VolTables& out);
+                      VolTables& out, const hipStream_t* device_stream = nullptr);
 
 }  // namespace grrt_amd

@@ -139,14 +139,20 @@ GRRT_EXPORT int grrt_ext_set_integrator(GRRTContext* ctx, int mode);
 /* grrt_render_spectral into a float32 cube (width*height*nbins). */
 GRRT_EXPORT int grrt_ext_render_spectral_f32(GRRTContext* ctx, float* out, int width, int height);
 
-/* Host-only access to the volumetric disk set-up (no GPU needed): builds
- * the opacity, radial and vertical tables for the volumetric disk that
- * grrt_create would build from `params`. */
+/* Access to the volumetric disk set-up (src/volumetric_disk.cpp:33-73,
+ * src/opacity.cpp:237-303): builds the opacity, radial and vertical tables
+ * for the volumetric disk that grrt_create would build from `params`, on
+ * host threads (no GPU needed). */
 typedef struct GRRTVolTables GRRTVolTables;
 GRRT_EXPORT GRRTVolTables* grrt_ext_vol_build(const GRRTParams* params);
+/* The same with the opacity tables (10^4 independent Saha solves and their
+ * absorption and Rosseland opacities) computed on the GPU, as grrt_create
+ * does; NULL without a GPU. */
+GRRT_EXPORT GRRTVolTables* grrt_ext_vol_build_device(const GRRTParams* params);
 /* Scalars in this order: r_isco, r_horizon, r_min, r_outer, taper_width,
  * E_isco, L_isco, rho_scale, noise_scale, turbulence, octaves, n_r, n_z,
- * n_nu, log10 nu_min, log10 nu_max, M, a, t_peak.  Returns the count. */
+ * n_nu, log10 nu_min, log10 nu_max, M, a, t_peak, then the set-up's wall
+ * time in ms: opacity tables, whole build.  Returns the count. */
 GRRT_EXPORT int grrt_ext_vol_scalars(const GRRTVolTables* t, double* out, int cap);
 /* Copies table `name` ("H", "z_max", "rho_mid", "T_eff", "rho_profile",
  * "T_profile", "kappa_abs", "kappa_es", "kappa_ross", "mu", "perm") into

@@ -270,3 +270,30 @@ def test_volumetric_step_doubling_parity(grrt, oracle):
         self_noise = _median_rel(frec["color"][same, c], orec["color"][same, c])
         assert g_vs_o <= max(20.0 * self_noise, 2e-5), (c, g_vs_o, self_noise)
     assert np.any(rec["cls"] == grrt.CLASS_DISK)
+
+
+def test_device_opacity_tables_equal_host(grrt):
+    """grrt_create computes the opacity tables on the GPU (one lane per
+    (rho, T) cell); the host builder runs the same point functions on host
+    threads.  The device exp/pow/log differ from the host's in the last bits,
+    so the tables agree to rounding, and so do the radial and vertical
+    structure built from them; the Saha passes' 2-cycles (partial
+    ionisation) are stable, so no cell drifts."""
+    print("case: opacity tables host vs GPU, spectral cube scene (60 frequencies) and default disk", flush=True)
+    for p in (grrt.make_params(width=64, height=64, metric=1, spin=0.9, volumetric=True, disk_outer=20.0,
+                               turbulence=0.4, seed=42, octaves=2, alpha=0.1, freqs=FREQS),
+              grrt.make_params(metric=1, spin=0.998, volumetric=True, disk_outer=30.0)):
+        host = grrt.volumetric_tables(p)
+        dev = grrt.volumetric_tables(p, device=True)
+        print(f"n_nu {int(host['n_nu'])}: opacity tables host {host['build_ms_opacity']:.1f} ms, "
+              f"GPU {dev['build_ms_opacity']:.1f} ms", flush=True)
+        for name in ("kappa_abs", "kappa_es", "kappa_ross", "mu"):
+            np.testing.assert_allclose(dev[name], host[name], rtol=1e-12, atol=1e-300, err_msg=name)
+        for name in ("H", "rho_mid", "T_eff", "z_max", "T_profile"):
+            np.testing.assert_allclose(dev[name], host[name], rtol=1e-9, atol=1e-300, err_msg=name)
+        # Density profiles are normalised to 1 at the midplane; the column's
+        # thin top (down to 1e-15) carries rounding of the opacities through
+        # the hydrostatic integration, so they agree to 1e-9 of the midplane.
+        np.testing.assert_allclose(dev["rho_profile"], host["rho_profile"], rtol=0, atol=1e-9)
+        assert dev["rho_scale"] == pytest.approx(host["rho_scale"], rel=1e-9)
+        assert np.array_equal(dev["perm"], host["perm"])
