@@ -85,7 +85,8 @@ GRRT_EXPORT int grrt_ext_get_frame(GRRTContext* ctx, float* out);
  * wavefront refill, and the rays a wavefront claims from the device queue
  * per atomic (>= 64); 0 keeps the current value.  tile: queue order of the
  * RGB frame, 0 = row-major samples, 2..32 = tile x tile pixel blocks;
- * negative keeps the current order. */
+ * negative keeps the current order.  Defaults: 256 threads, 3 blocks per
+ * CU, refill at 8 idle lanes, 64-ray claims, row-major. */
 GRRT_EXPORT int grrt_ext_set_tuning(GRRTContext* ctx, int block, int blocks_per_cu, int refill_min,
                                     int chunk, int tile);
 
