@@ -62,6 +62,11 @@ def test_no_gpu_fails_loudly():
         pytest.skip("a GPU is present")
     with pytest.raises(RuntimeError, match="grrt_create failed"):
         grrt_amd.Context(grrt_amd.make_params(width=8, height=8))
+    # the GPU table build refuses too; the host build needs no GPU
+    p = grrt_amd.make_params(metric=1, spin=0.9, volumetric=True, disk_outer=20.0)
+    with pytest.raises(RuntimeError, match="volumetric table build failed"):
+        grrt_amd.volumetric_tables(p, device=True)
+    assert grrt_amd.volumetric_tables(p)["n_nu"] == 20
 
 
 def test_tonemap_host_only():
