@@ -74,6 +74,7 @@ class GRRTStats(C.Structure):
         ("rays", C.c_int64), ("steps", C.c_int64), ("attempts", C.c_int64),
         ("deriv_evals", C.c_int64), ("algo_flops", C.c_double),
         ("cancelled", C.c_int32), ("pad", C.c_int32),
+        ("march_steps", C.c_int64), ("copy_ms", C.c_double),
     ]
 
 
@@ -179,6 +180,11 @@ API = [
     ("grrt_ext_surrogate_dataset", C.c_int, [_P, C.c_int, _P, _P, C.c_int64]),
     ("grrt_ext_render_surrogate", C.c_int, [_P, _P, C.c_int, C.c_double, C.c_double, C.c_double,
                                             C.POINTER(GRRTSurrogateStats)]),
+    ("grrt_ext_surrogate_infer_raw", C.c_int, [_P, _P, C.c_int64, _P]),
+    ("grrt_ext_surrogate_adam_step", C.c_int, [_P, _P, _P, _P, _P, C.c_int64, C.c_double, C.c_double,
+                                               C.c_double, C.c_double, C.c_int64]),
+    ("grrt_ext_surrogate_lists", C.c_int, [_P, _P, C.c_int64, _P, C.c_int64]),
+    ("grrt_ext_get_samples", C.c_int, [_P, _P, C.c_int64]),
     ("grrt_ext_tonemap_device", C.c_int, [_P, _P, C.POINTER(C.c_double)]),
     ("grrt_ext_get_frame", C.c_int, [_P, _P]),
     ("grrt_ext_set_integrator", C.c_int, [_P, C.c_int]),
@@ -188,6 +194,7 @@ API = [
     ("grrt_ext_vol_scalars", C.c_int, [_P, C.POINTER(C.c_double), C.c_int]),
     ("grrt_ext_vol_table", C.c_int64, [_P, C.c_char_p, C.POINTER(C.c_double), C.c_int64]),
     ("grrt_ext_vol_free", None, [_P]),
+    ("grrt_ext_resolve_params", C.c_int, [C.POINTER(GRRTParams), C.POINTER(C.c_double), C.c_int]),
     ("grrt_ext_saha", C.c_int, [C.c_double, C.c_double, C.POINTER(C.c_double)]),
     ("grrt_ext_opacity", C.c_int, [C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double)]),
 ]
@@ -276,6 +283,20 @@ def volumetric_tables(params: GRRTParams, device: bool = False) -> dict:
         return out
     finally:
         L.grrt_ext_vol_free(h)
+
+
+RESOLVED = ["width", "height", "M", "a", "r_obs", "theta_obs", "phi_obs", "fov", "max_steps", "tol",
+            "disk", "volumetric", "r_in", "r_out", "t_disk", "stars", "spp", "r_horizon", "r_isco",
+            "vol_M", "vol_spin", "vol_r_outer", "vol_t_peak", "vol_nu_min", "vol_nu_max"]
+
+
+def resolve_params(params: GRRTParams) -> dict:
+    """The scene grrt_create would build from params (no GPU needed)."""
+    out = np.zeros(len(RESOLVED))
+    n = lib().grrt_ext_resolve_params(C.byref(params), out.ctypes.data_as(C.POINTER(C.c_double)), len(out))
+    if n != len(RESOLVED):
+        raise RuntimeError("grrt_ext_resolve_params failed")
+    return dict(zip(RESOLVED, out.tolist()))
 
 
 def saha(rho, T) -> dict:
@@ -439,6 +460,41 @@ class Context:
         self._check(self._L.grrt_ext_render_surrogate(self._h, fb.ctypes.data, 0, entropy_max, lng_tol,
                                                        judge_fraction, C.byref(st)), "render_surrogate")
         return fb, _as_dict(st)
+
+    def surrogate_infer_raw(self, x: np.ndarray | None = None) -> np.ndarray:
+        """The surrogate frame's network pass, raw outputs [n][9]: on inputs x
+        [n][5], or on the frame's own samples (x None)."""
+        if x is None:
+            n, xp = self.num_samples(), None
+        else:
+            xx = np.ascontiguousarray(x, np.float32)
+            n, xp = xx.shape[0], xx.ctypes.data
+        y = np.zeros((n, 9), np.float32)
+        self._check(self._L.grrt_ext_surrogate_infer_raw(self._h, xp, n, y.ctypes.data), "surrogate_infer_raw")
+        return y
+
+    def surrogate_adam_step(self, w, g, m, v, lr, t, b1=0.9, b2=0.999, eps=1e-8):
+        """One step of the training loop's Adam kernel, in place on float32
+        arrays w, m, v."""
+        for a in (w, m, v):
+            assert a.dtype == np.float32 and a.flags.c_contiguous
+        gg = np.ascontiguousarray(g, np.float32)
+        self._check(self._L.grrt_ext_surrogate_adam_step(self._h, w.ctypes.data, gg.ctypes.data, m.ctypes.data,
+                                                          v.ctypes.data, w.size, lr, b1, b2, eps, t), "adam_step")
+
+    def surrogate_lists(self, n_fallback: int, n_judged: int):
+        fb = np.zeros(n_fallback, np.int64)
+        jd = np.zeros(n_judged, np.int64)
+        self._check(self._L.grrt_ext_surrogate_lists(self._h, fb.ctypes.data, n_fallback, jd.ctypes.data,
+                                                      n_judged), "surrogate_lists")
+        return fb, jd
+
+    def samples(self, n: int | None = None) -> np.ndarray:
+        """Per-sample linear RGB of the last frame, [n][3] float64."""
+        n = self.num_samples() if n is None else n
+        out = np.zeros((n, 3), np.float64)
+        self._check(self._L.grrt_ext_get_samples(self._h, out.ctypes.data, n), "get_samples")
+        return out
 
     def autotune(self, views=None) -> tuple[dict, float]:
         """Time launch shapes on the frame (views None) or the batch and keep

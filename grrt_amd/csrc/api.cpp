@@ -197,6 +197,44 @@ int samples_per_pixel(int mode, int spp, int& grid) {
     return std::max(spp, 1);
 }
 
+float live_progress(const GRRTContext* ctx) {
+    const double claimed = static_cast<double>(ctx->shared->claimed);
+    const double f = std::clamp(claimed / static_cast<double>(ctx->queue_len.load()), 0.0, 1.0);
+    const float p = ctx->progress_lo + (ctx->progress_hi - ctx->progress_lo) * static_cast<float>(f);
+    // Claim positions from different wavefronts can land out of order:
+    // report the largest value seen in this call (ctx->progress).
+    float seen = ctx->progress.load();
+    while (p > seen && !ctx->progress.compare_exchange_weak(seen, p)) {
+    }
+    return std::max(p, seen);
+}
+
+void begin_call(GRRTContext* ctx) {
+    *ctx->cancel_flag = 0;
+    ctx->progress.store(0.0f);
+    ctx->progress_lo = 0.0f;
+    ctx->progress_hi = 1.0f;
+}
+
+void frame_to_host(GRRTContext* ctx, float* host, const float* dev, size_t n) {
+    if (ctx->host_stage_cap < n) {
+        if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
+        ctx->host_stage = nullptr;
+        ctx->host_stage_cap = 0;
+        void* p = nullptr;
+        check(hipHostMalloc(&p, n * sizeof(float), hipHostMallocDefault), "hipHostMalloc(staging)");
+        ctx->host_stage = static_cast<float*>(p);
+        ctx->host_stage_cap = n;
+    }
+    check(hipMemcpyAsync(ctx->host_stage, dev, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream), "copy frame");
+    check(hipStreamSynchronize(ctx->stream), "copy frame");
+    if (!ctx->copy_pool) {
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        ctx->copy_pool = std::make_unique<grrt_amd::CopyPool>(static_cast<int>(std::clamp(hw / 2, 1u, 8u)) - 1);
+    }
+    ctx->copy_pool->copy(host, ctx->host_stage, n * sizeof(float));
+}
+
 enum class Output { kRGB, kSpectralF64, kSpectralF32 };
 
 // Render the pixel window [x0,x0+tw) x [y0,y0+th) into device memory `out`:
@@ -236,7 +274,8 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
     P.sample_cls = ctx->sample_cls.ptr;
     P.records = ctx->keep_records ? ctx->records.ptr : nullptr;
     P.counters = ctx->counters.ptr;
-    P.cancel = ctx->cancel_flag_dev;
+    P.cancel = &ctx->shared_dev->cancel;
+    P.claimed = &ctx->shared_dev->claimed;
     P.refill_min = std::clamp(ctx->tune_refill, 1, 64);
     P.chunk = std::max(ctx->tune_chunk, 64);
     P.integrator = ctx->integrator;
@@ -260,8 +299,11 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
         }
     }
 
-    *ctx->cancel_flag = 0;
-    ctx->progress.store(0.0f);
+    // The cancel flag is cleared once per API call (begin_call), not here:
+    // a request made between the bands of one call must stop the call.
+    ctx->shared->claimed = 0;
+    ctx->queue_len.store(std::max<long long>(P.n_rays, 1));
+    ctx->in_flight.store(1);
     const int threads = ctx->tune_block;
     const int blocks = std::max(1, ctx->num_cus * ctx->tune_blocks_per_cu);
     // The stream is idle here (every render ends with a synchronise), so the
@@ -277,20 +319,32 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
                "trace kernel launch");
     check(hipEventRecord(ctx->ev[1], ctx->stream), "hipEventRecord");
     if (spectral)
-        check(launch_finalize_spectral(P.sample_spec, d_out, npix, nS, nb, kind == Output::kSpectralF32, ctx->stream),
+        check(launch_finalize_spectral(P.sample_spec, P.sample_cls, d_out, npix, nS, nb, kind == Output::kSpectralF32,
+                                       ctx->stream),
               "finalize launch");
     else
         check(launch_finalize(P.sample_rgb, P.sample_cls, static_cast<float*>(d_out), npix, nS, ctx->stream),
               "finalize launch");
     check(hipEventRecord(ctx->ev[2], ctx->stream), "hipEventRecord");
 
-    // Wait on the caller's thread, reporting coarse progress from elapsed
-    // work; the exact count is read once the kernels end.
+    // Wait on the caller's thread.  With a callback, report the fraction of
+    // the ray queue claimed so far (the persistent kernels store each claim
+    // position into host-mapped memory), rising monotonically, as the
+    // reference's renderer reports finished rows (src/renderer.cpp:74-81).
     if (progress) {
-        while (hipEventQuery(ctx->ev[2]) == hipErrorNotReady)
-            std::this_thread::sleep_for(std::chrono::milliseconds(2));
+        float last = -1.0f;
+        while (hipEventQuery(ctx->ev[2]) == hipErrorNotReady) {
+            const float f = grrt_api::live_progress(ctx);
+            if (f > last) {
+                last = f;
+                progress(f, ud);
+            }
+            std::this_thread::sleep_for(std::chrono::microseconds(500));
+        }
     }
-    check(hipEventSynchronize(ctx->ev[2]), "render");
+    const hipError_t sync = hipEventSynchronize(ctx->ev[2]);
+    ctx->in_flight.store(0);
+    check(sync, "render");
     DevCounters c{};
     check(hipMemcpy(&c, P.counters, sizeof(c), hipMemcpyDeviceToHost), "read counters");
     float t_trace = 0.0f, t_fin = 0.0f;
@@ -308,12 +362,13 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
     s.deriv_evals = static_cast<int64_t>(c.rays_done + 6 * c.attempts);
     s.algo_flops = flops_per_attempt() * static_cast<double>(c.attempts);
     s.cancelled = cancelled ? 1 : 0;
+    s.march_steps = static_cast<int64_t>(c.march_steps);
     s.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     ctx->last_rays = n_rays;
     ctx->last_batch = false;
     ctx->records_valid = ctx->keep_records && !cancelled;
-    ctx->progress.store(1.0f);
-    if (progress) progress(1.0f, ud);
+    ctx->progress.store(ctx->progress_hi);
+    if (progress) progress(ctx->progress_hi, ud);
     return cancelled ? 1 : 0;
 }
 
@@ -325,6 +380,7 @@ int render_spectral_host(GRRTContext* ctx, void* host, int width, int height, bo
         return -1;
     }
     if (!host || width <= 0 || height <= 0) throw HipFailure("invalid spectral buffer");
+    begin_call(ctx);
     const size_t n = static_cast<size_t>(width) * height * ctx->frequency_bins.size();
     const size_t bytes = n * (f32 ? sizeof(float) : sizeof(double));
     if (!ctx->res.volumetric) {
@@ -410,9 +466,9 @@ int spectral_to_fits(GRRTContext* ctx, const char* path, int width, int height, 
         return -1;
     }
     if (width <= 0 || height <= 0) throw HipFailure("grrt_render_spectral_to_fits_cb: empty image");
+    begin_call(ctx);
     const int nb = static_cast<int>(ctx->frequency_bins.size());
     grrt_amd::FitsCubeWriter out(path, width, height, nb, ctx->frequency_bins, fits_meta(ctx->params), f32 ? -32 : -64);
-    ctx->progress.store(0.0f);
     int rc = 0;
     if (ctx->res.volumetric) {
         const size_t es = f32 ? sizeof(float) : sizeof(double);
@@ -421,7 +477,13 @@ int spectral_to_fits(GRRTContext* ctx, const char* path, int width, int height, 
         std::vector<unsigned char> host(static_cast<size_t>(width) * band * nb * es);
         check(ctx->spec_out.ensure(host.size()), "hipMalloc(spectral band)");
         for (int y0 = 0; y0 < height && rc == 0; y0 += band) {
+            // A cancel request (from another thread or from the progress
+            // callback between bands) ends the stream; the rows not yet
+            // written stay zero in the pre-extended file.
+            if (*ctx->cancel_flag) { rc = 1; break; }
             const int bh = std::min(band, height - y0);
+            ctx->progress_lo = static_cast<float>(y0) / static_cast<float>(height);
+            ctx->progress_hi = static_cast<float>(y0 + bh) / static_cast<float>(height);
             rc = render_window(ctx, 0, y0, width, bh, ctx->spec_out.ptr, nullptr, nullptr,
                                f32 ? Output::kSpectralF32 : Output::kSpectralF64);
             if (rc != 0) break;
@@ -441,7 +503,9 @@ int spectral_to_fits(GRRTContext* ctx, const char* path, int width, int height, 
         progress(1.0f, ud);
     }
     out.finish();
-    ctx->progress.store(1.0f);
+    ctx->progress_lo = 0.0f;
+    ctx->progress_hi = 1.0f;
+    if (rc == 0) ctx->progress.store(1.0f);
     return rc;
 }
 
@@ -468,13 +532,15 @@ GRRTContext* grrt_create(const GRRTParams* params) {
         check(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking), "hipStreamCreate");
         for (auto& e : ctx->ev) check(hipEventCreate(&e), "hipEventCreate");
         for (auto& e : ctx->band_ev) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
-        void* flag = nullptr;
-        check(hipHostMalloc(&flag, sizeof(int), hipHostMallocMapped | hipHostMallocCoherent), "hipHostMalloc");
-        ctx->cancel_flag = static_cast<volatile int*>(flag);
-        *ctx->cancel_flag = 0;
-        void* dflag = nullptr;
-        check(hipHostGetDevicePointer(&dflag, flag, 0), "hipHostGetDevicePointer");
-        ctx->cancel_flag_dev = static_cast<const volatile int*>(dflag);
+        void* shared = nullptr;
+        check(hipHostMalloc(&shared, sizeof(grrt_amd::HostShared), hipHostMallocMapped | hipHostMallocCoherent),
+              "hipHostMalloc");
+        std::memset(shared, 0, sizeof(grrt_amd::HostShared));
+        ctx->shared = static_cast<volatile grrt_amd::HostShared*>(shared);
+        ctx->cancel_flag = &ctx->shared->cancel;
+        void* dshared = nullptr;
+        check(hipHostGetDevicePointer(&dshared, shared, 0), "hipHostGetDevicePointer");
+        ctx->shared_dev = static_cast<grrt_amd::HostShared*>(dshared);
         void* staging = nullptr;
         check(hipHostMalloc(&staging, sizeof(grrt_amd::TraceParams), hipHostMallocDefault), "hipHostMalloc");
         ctx->params_host = static_cast<grrt_amd::TraceParams*>(staging);
@@ -528,7 +594,8 @@ void grrt_destroy(GRRTContext* ctx) {
     ctx->sample_spec.release();
     ctx->spec_out.release();
     if (ctx->params_host) (void)hipHostFree(ctx->params_host);
-    if (ctx->cancel_flag) (void)hipHostFree(const_cast<int*>(ctx->cancel_flag));
+    if (ctx->shared) (void)hipHostFree(const_cast<grrt_amd::HostShared*>(ctx->shared));
+    if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : ctx->band_ev)
@@ -557,10 +624,15 @@ int grrt_render_cb(GRRTContext* ctx, float* framebuffer, grrt_progress_fn progre
         if (!framebuffer) throw HipFailure("framebuffer is null");
         const auto& r = ctx->res;
         const size_t npix = static_cast<size_t>(r.W) * r.H;
+        const auto t0 = std::chrono::steady_clock::now();
+        begin_call(ctx);
         check(ctx->frame.ensure(4 * npix), "hipMalloc(frame)");
         const int rc = render_window(ctx, 0, 0, r.W, r.H, ctx->frame.ptr, progress, user_data);
-        check(hipMemcpy(framebuffer, ctx->frame.ptr, 4 * npix * sizeof(float), hipMemcpyDeviceToHost),
-              "copy frame");
+        const auto t1 = std::chrono::steady_clock::now();
+        frame_to_host(ctx, framebuffer, ctx->frame.ptr, 4 * npix);
+        const auto t2 = std::chrono::steady_clock::now();
+        ctx->last.copy_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+        ctx->last.total_ms = std::chrono::duration<double, std::milli>(t2 - t0).count();
         return rc;
     });
 }
@@ -572,10 +644,10 @@ int grrt_render_tile(GRRTContext* ctx, float* buffer, int x, int y, int tile_wid
             y + tile_height > r.H)
             throw HipFailure("grrt_render_tile: tile outside the frame");
         const size_t npix = static_cast<size_t>(tile_width) * tile_height;
+        begin_call(ctx);
         check(ctx->frame.ensure(4 * npix), "hipMalloc(frame)");
         const int rc = render_window(ctx, x, y, tile_width, tile_height, ctx->frame.ptr, nullptr, nullptr);
-        check(hipMemcpy(buffer, ctx->frame.ptr, 4 * npix * sizeof(float), hipMemcpyDeviceToHost),
-              "copy tile");
+        frame_to_host(ctx, buffer, ctx->frame.ptr, 4 * npix);
         return rc;
     });
 }
@@ -584,7 +656,10 @@ void grrt_cancel(GRRTContext* ctx) {
     if (ctx && ctx->cancel_flag) *ctx->cancel_flag = 1;
 }
 
-float grrt_progress(const GRRTContext* ctx) { return ctx ? ctx->progress.load() : 0.0f; }
+float grrt_progress(const GRRTContext* ctx) {
+    if (!ctx) return 0.0f;
+    return ctx->in_flight.load() ? grrt_api::live_progress(ctx) : ctx->progress.load();
+}
 
 const char* grrt_error(const GRRTContext* ctx) {
     return (!ctx || ctx->error_msg.empty()) ? nullptr : ctx->error_msg.c_str();
@@ -610,6 +685,7 @@ void grrt_debug_pixel(GRRTContext* ctx, int px, int py) {
         const bool keep = ctx->keep_records;
         ctx->sampling_mode = GRRT_SAMPLING_CENTER;
         ctx->keep_records = true;
+        begin_call(ctx);
         check(ctx->frame.ensure(4), "hipMalloc(frame)");
         render_window(ctx, px, py, 1, 1, ctx->frame.ptr, nullptr, nullptr);
         ctx->sampling_mode = mode;
@@ -683,8 +759,9 @@ int grrt_ext_get_records(GRRTContext* ctx, GRRTRayRecord* out, int64_t n) {
     return guarded(ctx, [&] {
         if (!ctx->records_valid) throw HipFailure("no records: enable them before rendering");
         if (!out || n < 0 || n > ctx->last_rays) throw HipFailure("record count out of range");
-        check(hipMemcpy(out, ctx->records.ptr, static_cast<size_t>(n) * sizeof(GRRTRayRecord),
-                        hipMemcpyDeviceToHost), "copy records");
+        const GRRTRayRecord* src = ctx->last_batch ? ctx->batch_records.ptr : ctx->records.ptr;
+        check(hipMemcpy(out, src, static_cast<size_t>(n) * sizeof(GRRTRayRecord), hipMemcpyDeviceToHost),
+              "copy records");
         return 0;
     });
 }
@@ -694,6 +771,15 @@ int grrt_ext_get_classes(GRRTContext* ctx, uint8_t* out, int64_t n) {
         if (!out || n < 0 || n > ctx->last_rays) throw HipFailure("class count out of range");
         const uint8_t* src = ctx->last_batch ? ctx->batch_cls.ptr : ctx->sample_cls.ptr;
         check(hipMemcpy(out, src, static_cast<size_t>(n), hipMemcpyDeviceToHost), "copy classes");
+        return 0;
+    });
+}
+
+int grrt_ext_get_samples(GRRTContext* ctx, double* rgb, int64_t n) {
+    return guarded(ctx, [&] {
+        if (!rgb || n < 0 || n > ctx->last_rays) throw HipFailure("sample count out of range");
+        const double* src = ctx->last_batch ? ctx->batch_rgb.ptr : ctx->sample_rgb.ptr;
+        check(hipMemcpy(rgb, src, static_cast<size_t>(n) * 3 * sizeof(double), hipMemcpyDeviceToHost), "copy samples");
         return 0;
     });
 }
@@ -712,6 +798,7 @@ int grrt_ext_render_device(GRRTContext* ctx, float* d_rgba) {
             check(ctx->frame.ensure(4 * static_cast<size_t>(r.W) * r.H), "hipMalloc(frame)");
             dst = ctx->frame.ptr;
         }
+        begin_call(ctx);
         return render_window(ctx, 0, 0, r.W, r.H, dst, nullptr, nullptr);
     });
 }
@@ -877,6 +964,25 @@ int64_t grrt_ext_vol_table(const GRRTVolTables* h, const char* name, double* out
 
 void grrt_ext_vol_free(GRRTVolTables* h) { delete h; }
 
+int grrt_ext_resolve_params(const GRRTParams* params, double* out, int cap) {
+    if (!params || !out) return -1;
+    try {
+        const grrt_amd::Resolved r = grrt_amd::resolve_params(*params);
+        const VolSetup vs = resolve_volumetric(*params);
+        const double v[] = {static_cast<double>(r.W), static_cast<double>(r.H), r.M, r.a, r.r_obs, r.th_obs,
+                            r.ph_obs, r.fov, static_cast<double>(r.max_steps), r.tol, static_cast<double>(r.disk),
+                            static_cast<double>(r.volumetric), r.r_in, r.r_out, r.t_disk,
+                            static_cast<double>(r.stars), static_cast<double>(r.spp), r.r_h, r.r_isco, vs.M, vs.a,
+                            vs.r_outer, vs.t_peak, vs.vp.opacity_nu_min, vs.vp.opacity_nu_max};
+        const int n = static_cast<int>(sizeof(v) / sizeof(v[0]));
+        for (int i = 0; i < n && i < cap; ++i) out[i] = v[i];
+        return n;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return -1;
+    }
+}
+
 int grrt_ext_saha(double rho, double T, double out[5]) {
     if (!out) return -1;
     const grrt_amd::Ionization s = grrt_amd::saha(rho, T);
@@ -910,14 +1016,27 @@ void prepare_batch(GRRTContext* ctx, const GRRTView* views, int n, uint64_t seed
     using namespace grrt_amd;
     const Resolved& r = ctx->res;
     if (r.volumetric) throw HipFailure("batched rendering covers the thin disk only");
+    // Every view is checked before anything is freed or re-recorded, so a
+    // rejected batch leaves the previous recording (graph, views, tables
+    // and buffers) intact and replayable.
+    for (int v = 0; v < n; ++v) {
+        const GRRTView& w = views[v];
+        if (w.width <= 0 || w.height <= 0) throw HipFailure("batch view with an empty frame");
+        if (!(std::abs(w.spin) < 1.0)) throw HipFailure("batch view spin must satisfy |a/M| < 1");
+    }
+    // The recorded graph holds pointers into the buffers grown below: drop
+    // it (and the views it was recorded for) before any of them moves.
+    if (ctx->batch_exec) (void)hipGraphExecDestroy(ctx->batch_exec);
+    if (ctx->batch_graph) (void)hipGraphDestroy(ctx->batch_graph);
+    ctx->batch_exec = nullptr;
+    ctx->batch_graph = nullptr;
+    ctx->batch_views.clear();
     std::vector<ViewParams> vp(n);
     std::vector<double> flux(static_cast<size_t>(n) * kFluxN, 0.0);
     check(ctx->batch_flux.ensure(flux.size()), "hipMalloc(batch flux)");
     long long q = 0, rays = 0, pix = 0;
     for (int v = 0; v < n; ++v) {
         const GRRTView& w = views[v];
-        if (w.width <= 0 || w.height <= 0) throw HipFailure("batch view with an empty frame");
-        if (!(std::abs(w.spin) < 1.0)) throw HipFailure("batch view spin must satisfy |a/M| < 1");
         const double a = w.spin * r.M;
         const double r_obs = w.observer_r > 0.0 ? w.observer_r : r.r_obs;
         const double th = w.observer_theta > 0.0 ? w.observer_theta : r.th_obs;
@@ -972,11 +1091,6 @@ void prepare_batch(GRRTContext* ctx, const GRRTView* views, int n, uint64_t seed
     P.integrator = ctx->integrator;
     check(hipMemcpy(ctx->batch_tp.ptr, &P, sizeof(TraceParams), hipMemcpyHostToDevice), "hipMemcpy(batch params)");
     // Record: reset, the persistent trace over all views, finalize.
-    if (ctx->batch_exec) (void)hipGraphExecDestroy(ctx->batch_exec);
-    if (ctx->batch_graph) (void)hipGraphDestroy(ctx->batch_graph);
-    ctx->batch_exec = nullptr;
-    ctx->batch_graph = nullptr;
-    ctx->batch_views.clear();
     check(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
     hipError_t e = launch_reset(P.counters, P.sample_cls, rays, ctx->stream);
     if (e == hipSuccess)
@@ -1029,7 +1143,7 @@ int run_batch(GRRTContext* ctx) {
     s.algo_flops = flops_per_attempt() * static_cast<double>(c.attempts);
     s.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     ctx->last_rays = ctx->batch_rays;
-    ctx->records_valid = false;
+    ctx->records_valid = ctx->keep_records && ctx->batch_records.ptr != nullptr;
     ctx->last_batch = true;
     return 0;
 }
@@ -1064,6 +1178,7 @@ int grrt_ext_autotune(GRRTContext* ctx, const GRRTView* views, int n, int best[4
         const auto& r = ctx->res;
         if (r.volumetric) throw HipFailure("autotune covers the thin-disk tracer");
         if (!views) check(ctx->frame.ensure(4 * static_cast<size_t>(r.W) * r.H), "hipMalloc(frame)");
+        begin_call(ctx);
         for (int tile : tiles)
             for (int bpc : bpcs)
                 for (int rf : refills)

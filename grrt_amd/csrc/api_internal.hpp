@@ -52,6 +52,18 @@ void fill_view(const GRRTContext* ctx, grrt_amd::ViewParams& V, double a, double
                double flux_max, double r_in, int x0, int y0, int tw, int th, int tile);
 int samples_per_pixel(int mode, int spp, int& grid);
 
+// Progress of the call in flight: the claimed fraction of the current
+// launch's ray queue, mapped into [progress_lo, progress_hi] of the call and
+// never below the last value reported.
+float live_progress(const GRRTContext* ctx);
+
+// Start of a rendering API call: clears the cancel request and the progress.
+void begin_call(GRRTContext* ctx);
+
+// Copies a device frame of n floats to the caller's host buffer through the
+// context's pinned staging buffer (one DMA copy, then a threaded host copy).
+void frame_to_host(GRRTContext* ctx, float* host, const float* dev, size_t n);
+
 // Batched views: record (views -> graph) and replay.  seed_stride != 0
 // gives view v the seeded-jitter key seed + v * seed_stride (default: all
 // views share the context's key, as a single-view render would use).

@@ -5,6 +5,7 @@
 
 #include <atomic>
 #include <cstdint>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -13,6 +14,7 @@
 #include "scene_host.hpp"
 #include "volumetric_host.hpp"
 #include "trace_params.hpp"
+#include "copy_pool.hpp"
 
 namespace grrt_amd {
 
@@ -44,8 +46,8 @@ hipError_t launch_reset(DevCounters* c, uint8_t* cls, long long n_rays, hipStrea
 int vol_group_size(int nbins);
 hipError_t launch_trace_vol(const TraceParams* P, long long n_rays, int nbins, bool spectral, bool fp32_rt,
                             hipStream_t st);
-hipError_t launch_finalize_spectral(const double* spec, void* out, long long npix, int nS, int nb, bool f32,
-                                    hipStream_t st);
+hipError_t launch_finalize_spectral(const double* spec, const uint8_t* cls, void* out, long long npix, int nS,
+                                    int nb, bool f32, hipStream_t st);
 // mode: 0 one view, 1 batch of views, 2 listed rays of one view.
 // integrator: 0 adaptive Dormand-Prince, 1 step-doubling RK4.
 hipError_t launch_trace_thin(const TraceParams* P, int mode, int blocks, int threads, hipStream_t st,
@@ -103,8 +105,16 @@ struct GRRTContext {
     grrt_amd::DevBuf<grrt_amd::DevCounters> counters;
     grrt_amd::DevBuf<grrt_amd::TraceParams> params_dev;   // launch parameters
     grrt_amd::TraceParams* params_host = nullptr;          // pinned staging copy
-    volatile int* cancel_flag = nullptr;     // host-mapped, polled by the kernel
-    const volatile int* cancel_flag_dev = nullptr;   // device alias of cancel_flag
+    // Host-mapped block: the cancel flag the kernels poll at each queue
+    // claim, and the claim position they report back.
+    volatile grrt_amd::HostShared* shared = nullptr;
+    grrt_amd::HostShared* shared_dev = nullptr;      // device alias of shared
+    volatile int* cancel_flag = nullptr;             // &shared->cancel
+    // A render in flight: grrt_progress reads shared->claimed against
+    // queue_len (progress_lo..progress_hi of the whole call).
+    std::atomic<int> in_flight{0};
+    std::atomic<long long> queue_len{1};
+    float progress_lo = 0.0f, progress_hi = 1.0f;
 
     // Sampling and tuning.
     int sampling_mode = GRRT_SAMPLING_HASH;
@@ -144,7 +154,13 @@ struct GRRTContext {
     // Learned surrogate (network, optimiser state, datasets), created on demand.
     grrt_amd::sg::SurrogateState* sg = nullptr;
 
+    // Pinned host staging for frames returned to a caller's host buffer:
+    // one DMA copy from the device frame, then a threaded host copy.
+    float* host_stage = nullptr;
+    size_t host_stage_cap = 0;          // floats
+    std::unique_ptr<grrt_amd::CopyPool> copy_pool;
+
     GRRTStats last{};
     long long last_rays = 0;
-    std::atomic<float> progress{1.0f};
+    mutable std::atomic<float> progress{1.0f};   // highest progress reported in the current call
 };

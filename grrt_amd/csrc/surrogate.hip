@@ -399,14 +399,20 @@ __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
             asm volatile("" ::: "memory");
             long long i = ray0 + lane;
             if (i >= n) i = n - 1;
-            SampleId d;
-            decode_ray(V, visit_ray(IA, i), d);
-            double px, py;
-            sample_position(V, d, px, py);
-            const double alpha = (px - V.W / 2.0) * V.fov / V.W;
-            const double beta = (py - V.H / 2.0) * V.fov / V.W;
             float xl[kIn];
-            features(alpha, beta, V.a / V.M, cos(V.th_obs), V.r_obs / V.M, xl);
+            if (IA.x_in) {
+                const long long vi = visit_ray(IA, i);
+#pragma unroll
+                for (int k = 0; k < kIn; ++k) xl[k] = IA.x_in[vi * kIn + k];
+            } else {
+                SampleId d;
+                decode_ray(V, visit_ray(IA, i), d);
+                double px, py;
+                sample_position(V, d, px, py);
+                const double alpha = (px - V.W / 2.0) * V.fov / V.W;
+                const double beta = (py - V.H / 2.0) * V.fov / V.W;
+                features(alpha, beta, V.a / V.M, cos(V.th_obs), V.r_obs / V.M, xl);
+            }
 #pragma unroll
             for (int k = 0; k < kIn; ++k) s_x[wave][lane][k] = xl[k];
             __builtin_amdgcn_wave_barrier();
@@ -516,6 +522,15 @@ __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
 #pragma unroll
         for (int m = 0; m < kOut; ++m) y[m] = outs[wave][lane][m];
         __builtin_amdgcn_wave_barrier();
+        if (IA.y_raw) {
+            if (valid)
+#pragma unroll
+                for (int m = 0; m < kOut; ++m) IA.y_raw[i * kOut + m] = y[m];
+            const long long nxt = s_claim;
+            __syncthreads();
+            tile = nxt;
+            continue;
+        }
         const float mx = fmaxf(y[0], fmaxf(y[1], y[2]));
         const float e0 = __expf(y[0] - mx), e1 = __expf(y[1] - mx), e2 = __expf(y[2] - mx);
         const float inv = 1.0f / (e0 + e1 + e2);

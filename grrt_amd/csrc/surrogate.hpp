@@ -113,6 +113,12 @@ struct InferArgs {
     unsigned long long* counts;         // [0] fallbacks, [1] judged
     long long judge_cap;
     unsigned long long* tile_counter;   // dynamic 256-ray tile claims (zeroed before the launch)
+    // Inspection of the network pass itself: with x_in the inputs of ray i
+    // are x_in[i][5] instead of the ones computed from its sample position;
+    // with y_raw the pass writes the network's 9 outputs of ray i to
+    // y_raw[i][9] and neither shades nor lists any ray.
+    const float* x_in;
+    float* y_raw;
 };
 
 hipError_t launch_pack(const float* params, __bf16* pack0, __bf16* packH, __bf16* pack4, __bf16* WT,

@@ -33,6 +33,7 @@ struct SurrogateState {
     DevBuf<unsigned long long> counts;
     DevBuf<float> judge_err;
     uint32_t frame_no = 0;
+    unsigned long long last_fb = 0, last_judged = 0;   // list lengths of the last surrogate frame
 
     ~SurrogateState() {
         for (auto* b : {&w, &m, &v, &grad, &X, &Xv, &Y, &ZT, &dY, &dHT, &part, &dbp, &judge_err}) b->release();
@@ -156,9 +157,117 @@ void fill_stats(const double* acc, LossWeights lw, GRRTTrainStats* out) {
     out->loss = (acc[0] + lw.w_r * acc[2] + lw.w_g * acc[3] + lw.w_dir * acc[5]) / n;
 }
 
+// The frame's trace parameters (one view: the context's camera, disk and
+// sampling) for the network pass and the fallback trace.
+TraceParams frame_params(GRRTContext* ctx, long long n_rays) {
+    const Resolved& r = ctx->res;
+    TraceParams P{};
+    grrt_api::fill_view(ctx, P.view, r.a, r.r_obs, r.th_obs, r.ph_obs, r.fov, r.W, r.H, ctx->cam, ctx->flux.ptr,
+                        ctx->flux_max, r.r_in, 0, 0, r.W, r.H, 0);
+    P.r_escape = 1000.0;
+    P.tol = r.tol;
+    P.max_steps = r.max_steps;
+    P.n_rays = n_rays;
+    P.view.n_rays = n_rays;
+    P.spec_lut = ctx->spec.ptr;
+    P.stars = r.stars;
+    P.star_start = ctx->star_start.ptr;
+    P.star_tpb = ctx->star_tpb.ptr;
+    P.sample_rgb = ctx->sample_rgb.ptr;
+    P.sample_cls = ctx->sample_cls.ptr;
+    P.records = nullptr;
+    P.cancel = nullptr;
+    P.refill_min = std::clamp(ctx->tune_refill, 1, 64);
+    P.integrator = ctx->integrator;
+    return P;
+}
+
 }  // namespace
 
 extern "C" {
+
+int grrt_ext_surrogate_infer_raw(GRRTContext* ctx, const float* x, int64_t n, float* y) {
+    return grrt_api::guarded(ctx, [&] {
+        SurrogateState& S = state(ctx);
+        if (!y || n <= 0) throw HipFailure("surrogate raw pass: no rays");
+        const Resolved& r = ctx->res;
+        int grid = 1;
+        const int nS = grrt_api::samples_per_pixel(ctx->sampling_mode, ctx->sampling_spp, grid);
+        const long long frame_rays = static_cast<long long>(r.W) * r.H * nS;
+        if (!x && n != frame_rays) throw HipFailure("surrogate raw pass: without inputs n must be the frame's ray count");
+        check(ctx->params_dev.ensure(1), "hipMalloc(params)");
+        check(S.counts.ensure(4), "hipMalloc(counts)");
+        DevBuf<float> dx, dy;
+        if (x) {
+            check(dx.ensure(static_cast<size_t>(n) * kIn), "hipMalloc(x)");
+            check(hipMemcpy(dx.ptr, x, sizeof(float) * kIn * n, hipMemcpyHostToDevice), "copy x");
+        }
+        check(dy.ensure(static_cast<size_t>(n) * kOut), "hipMalloc(y)");
+        const TraceParams P = frame_params(ctx, x ? n : frame_rays);
+        check(hipMemcpy(ctx->params_dev.ptr, &P, sizeof(P), hipMemcpyHostToDevice), "upload params");
+        check(launch_reset_counts(S.counts.ptr, 4, ctx->stream), "reset counts");
+        InferArgs IA{};
+        IA.mlp = net_args(S);
+        IA.mlp.n = n;
+        IA.P = ctx->params_dev.ptr;
+        IA.entropy_max = 1e30;
+        IA.ray_base = 0;
+        IA.row_rays = 0;
+        IA.rows = r.H;
+        IA.counts = S.counts.ptr;
+        IA.tile_counter = S.counts.ptr + 3;
+        IA.x_in = x ? dx.ptr : nullptr;
+        IA.y_raw = dy.ptr;
+        check(launch_mlp_infer(IA, ctx->num_cus, ctx->stream), "surrogate pass");
+        check(hipStreamSynchronize(ctx->stream), "surrogate pass");
+        check(hipMemcpy(y, dy.ptr, sizeof(float) * kOut * n, hipMemcpyDeviceToHost), "copy y");
+        dx.release();
+        dy.release();
+        return 0;
+    });
+}
+
+int grrt_ext_surrogate_adam_step(GRRTContext* ctx, float* w, const float* g, float* m, float* v, int64_t n,
+                                 double lr, double b1, double b2, double eps, int64_t t) {
+    return grrt_api::guarded(ctx, [&] {
+        if (!w || !g || !m || !v || n <= 0 || t <= 0) throw HipFailure("adam step: bad arguments");
+        DevBuf<float> dw, dg, dm, dv;
+        const size_t bytes = sizeof(float) * static_cast<size_t>(n);
+        for (auto* b : {&dw, &dg, &dm, &dv}) check(b->ensure(static_cast<size_t>(n)), "hipMalloc(adam)");
+        check(hipMemcpy(dw.ptr, w, bytes, hipMemcpyHostToDevice), "copy w");
+        check(hipMemcpy(dg.ptr, g, bytes, hipMemcpyHostToDevice), "copy g");
+        check(hipMemcpy(dm.ptr, m, bytes, hipMemcpyHostToDevice), "copy m");
+        check(hipMemcpy(dv.ptr, v, bytes, hipMemcpyHostToDevice), "copy v");
+        // The bias corrections exactly as the training loop forms them.
+        check(launch_adam(dw.ptr, dg.ptr, dm.ptr, dv.ptr, n, static_cast<float>(lr), static_cast<float>(b1),
+                          static_cast<float>(b2), static_cast<float>(eps),
+                          static_cast<float>(1.0 - std::pow(b1, static_cast<double>(t))),
+                          static_cast<float>(1.0 - std::pow(b2, static_cast<double>(t))), ctx->stream),
+              "adam");
+        check(hipStreamSynchronize(ctx->stream), "adam");
+        check(hipMemcpy(w, dw.ptr, bytes, hipMemcpyDeviceToHost), "copy w");
+        check(hipMemcpy(m, dm.ptr, bytes, hipMemcpyDeviceToHost), "copy m");
+        check(hipMemcpy(v, dv.ptr, bytes, hipMemcpyDeviceToHost), "copy v");
+        for (auto* b : {&dw, &dg, &dm, &dv}) b->release();
+        return 0;
+    });
+}
+
+int grrt_ext_surrogate_lists(GRRTContext* ctx, int64_t* fallback, int64_t n_fallback, int64_t* judged,
+                             int64_t n_judged) {
+    return grrt_api::guarded(ctx, [&] {
+        SurrogateState& S = state(ctx);
+        if (n_fallback < 0 || n_judged < 0 || n_fallback > static_cast<int64_t>(S.last_fb) ||
+            n_judged > static_cast<int64_t>(S.last_judged))
+            throw HipFailure("surrogate lists: count out of range");
+        static_assert(sizeof(long long) == sizeof(int64_t), "ray ids are 64-bit");
+        if (fallback && n_fallback)
+            check(hipMemcpy(fallback, S.fb_list.ptr, sizeof(int64_t) * n_fallback, hipMemcpyDeviceToHost), "fallback ids");
+        if (judged && n_judged)
+            check(hipMemcpy(judged, S.judge_list.ptr, sizeof(int64_t) * n_judged, hipMemcpyDeviceToHost), "judged ids");
+        return 0;
+    });
+}
 
 int grrt_ext_surrogate_init(GRRTContext* ctx, uint64_t seed) {
     return grrt_api::guarded(ctx, [&] {
@@ -469,23 +578,7 @@ int grrt_ext_render_surrogate(GRRTContext* ctx, float* out, int device_out, doub
         check(S.judge_err.ensure(static_cast<size_t>(cap)), "hipMalloc(judge errors)");
         check(S.counts.ensure(static_cast<size_t>(4 * nb)), "hipMalloc(counts)");
 
-        TraceParams P{};
-        grrt_api::fill_view(ctx, P.view, r.a, r.r_obs, r.th_obs, r.ph_obs, r.fov, r.W, r.H, ctx->cam, ctx->flux.ptr,
-                            ctx->flux_max, r.r_in, 0, 0, r.W, r.H, 0);
-        P.r_escape = 1000.0;
-        P.tol = r.tol;
-        P.max_steps = r.max_steps;
-        P.n_rays = n_rays;
-        P.view.n_rays = n_rays;
-        P.spec_lut = ctx->spec.ptr;
-        P.stars = r.stars;
-        P.star_start = ctx->star_start.ptr;
-        P.star_tpb = ctx->star_tpb.ptr;
-        P.sample_rgb = ctx->sample_rgb.ptr;
-        P.sample_cls = ctx->sample_cls.ptr;
-        P.records = nullptr;
-        P.cancel = nullptr;
-        P.refill_min = std::clamp(ctx->tune_refill, 1, 64);
+        TraceParams P = frame_params(ctx, n_rays);
         // One wavefront's worth of listed rays per queue claim: every lane of
         // the launch starts on a ray at once, so the longest integrations at
         // the head of the list are not left waiting for a refill (3.2 ms
@@ -571,6 +664,8 @@ int grrt_ext_render_surrogate(GRRTContext* ctx, float* out, int device_out, doub
                                 hipMemcpyDeviceToHost),
                       "judge");
         }
+        S.last_fb = n_fb;
+        S.last_judged = nj;
         if (out)
             check(hipMemcpy(out, ctx->frame.ptr, sizeof(float) * 4 * npix,
                             device_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost),

@@ -24,7 +24,16 @@ struct DevCounters {
     unsigned long long attempts;
     unsigned long long derivs;
     unsigned long long rays_done;
-    unsigned long long pad[3];
+    unsigned long long march_steps;   // volumetric ray-march RK4 steps
+    unsigned long long pad[2];
+};
+
+// Host-mapped, coherent block per context: written by the host (cancel) or
+// the device (claimed), read by the other side while a render runs.
+struct HostShared {
+    int cancel;
+    int pad;
+    unsigned long long claimed;
 };
 
 // Pointers into device memory plus the scalars of one volumetric disk.
@@ -110,6 +119,9 @@ struct TraceParams {
     GRRTRayRecord* records;             // n_rays or nullptr
     DevCounters* counters;
     const volatile int* cancel;         // host-mapped flag, may be nullptr
+    // Host-mapped: the first ray of the latest claimed chunk, stored by the
+    // claiming lane (a progress reading for the host; may be nullptr).
+    unsigned long long* claimed;
 
     // Listed rays (trace mode 2): fallback ids, then judged ids whose results
     // are written after the view's n_rays samples; counts in device memory.

@@ -261,6 +261,7 @@ void trace_thin(const TraceParams* Pg) {
     const unsigned long long nR = static_cast<unsigned long long>(n_queue);
     const unsigned long long CH = static_cast<unsigned long long>(chunk);
     const volatile int* cancel = Pg->cancel;
+    unsigned long long* const claimed = Pg->claimed;
     unsigned long long cur = 0, cur_end = 0;     // wave-uniform claimed range
     unsigned long long pre = 0;                  // lane 0: base of the claim in flight
     int cancel_seen = 0;                         // lane 0: cancel flag read with that claim
@@ -283,6 +284,11 @@ void trace_thin(const TraceParams* Pg) {
                 const unsigned long long got = __shfl(pre, 0);
                 const int stop = __shfl(cancel_seen, 0);
                 have_pre = false;
+                // Progress for the host: a plain system-scope store of the
+                // claim position (stores from different waves may land out
+                // of order; the host keeps the largest value it has seen).
+                if (lane == 0 && claimed)
+                    __hip_atomic_store(claimed, got, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
                 if (got < nR) {
                     nb = got;
                     if (!stop) {

@@ -487,6 +487,9 @@ __global__ __launch_bounds__(256) void trace_vol(const TraceParams* Pg, int grou
     const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     const long long q = kKind != kBinsGlobal ? t / group : t;
     if (q >= P.n_rays) return;
+    // A cancelled render starts no new ray; the ray's samples keep the
+    // untraced class marker and finalize writes its pixel as zero.
+    if (P.cancel && *P.cancel) return;
     const VolDev& V = P.vol;
     const ViewParams& Vw = P.view;
     const long long id = centre_out_ray(Vw, q);
@@ -612,18 +615,22 @@ __global__ __launch_bounds__(256) void trace_vol(const TraceParams* Pg, int grou
     atomicAdd(&P.counters->steps, static_cast<unsigned long long>(steps));
     atomicAdd(&P.counters->attempts, static_cast<unsigned long long>(attempts));
     atomicAdd(&P.counters->rays_done, 1ull);
+    atomicAdd(&P.counters->march_steps, static_cast<unsigned long long>(march_steps));
 }
 
 // Average the N-bin spectra of each pixel (src/renderer.cpp:235-266); the
 // output is double (ABI) or float (cube) per `out_f32`.
-__global__ void finalize_spectral(const double* __restrict__ spec, void* out, long long npix, int nS,
-                                  int nb, double inv, int out_f32) {
+__global__ void finalize_spectral(const double* __restrict__ spec, const uint8_t* __restrict__ cls, void* out,
+                                  long long npix, int nS, int nb, double inv, int out_f32) {
     const long long q = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (q >= npix * nb) return;
     const long long p = q / nb;
     const int k = static_cast<int>(q - p * nb);
+    bool done = true;                 // a cancelled render leaves untraced samples (class 0xFF): zero
+    for (int s = 0; s < nS; ++s) done = done && cls[p * nS + s] != 0xFF;
     double acc = 0.0;
-    for (int s = 0; s < nS; ++s) acc += spec[(p * nS + s) * nb + k];
+    if (done)
+        for (int s = 0; s < nS; ++s) acc += spec[(p * nS + s) * nb + k];
     if (out_f32) static_cast<float*>(out)[q] = static_cast<float>(acc * inv);
     else static_cast<double*>(out)[q] = acc * inv;
 }
@@ -659,12 +666,12 @@ hipError_t launch_trace_vol(const TraceParams* P, long long n_rays, int nbins, b
     return hipGetLastError();
 }
 
-hipError_t launch_finalize_spectral(const double* spec, void* out, long long npix, int nS, int nb, bool f32,
-                                    hipStream_t st) {
+hipError_t launch_finalize_spectral(const double* spec, const uint8_t* cls, void* out, long long npix, int nS,
+                                    int nb, bool f32, hipStream_t st) {
     const int T = 256;
     const long long n = npix * nb;
     hipLaunchKernelGGL(finalize_spectral, dim3(static_cast<unsigned>((n + T - 1) / T)), dim3(T), 0, st, spec,
-                       out, npix, nS, nb, 1.0 / nS, f32 ? 1 : 0);
+                       cls, out, npix, nS, nb, 1.0 / nS, f32 ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -55,6 +55,8 @@ typedef struct {
     double algo_flops;     /* algorithmic FP64 flops of the trace kernel */
     int32_t cancelled;
     int32_t pad;
+    int64_t march_steps;   /* volumetric ray-march RK4 steps */
+    double copy_ms;        /* device-to-host copy of the frame (grrt_render*), host wall clock */
 } GRRTStats;
 
 GRRT_EXPORT int grrt_ext_set_sampling(GRRTContext* ctx, int mode, int spp, uint64_t seed);
@@ -65,6 +67,9 @@ GRRT_EXPORT int grrt_ext_set_records(GRRTContext* ctx, int enable);
 GRRT_EXPORT int64_t grrt_ext_num_samples(const GRRTContext* ctx);
 GRRT_EXPORT int grrt_ext_get_records(GRRTContext* ctx, GRRTRayRecord* out, int64_t n);
 GRRT_EXPORT int grrt_ext_get_classes(GRRTContext* ctx, uint8_t* out, int64_t n);
+/* Per-sample linear RGB of the last frame (n <= grrt_ext_num_samples, three
+ * doubles per sample, pixel-major sample order). */
+GRRT_EXPORT int grrt_ext_get_samples(GRRTContext* ctx, double* rgb, int64_t n);
 GRRT_EXPORT int grrt_ext_last_stats(const GRRTContext* ctx, GRRTStats* out);
 /* Render the full frame into device memory (d_rgba: width*height*4 floats,
  * or NULL to keep it in the context).  No host copy is made. */
@@ -160,6 +165,14 @@ GRRT_EXPORT int grrt_ext_vol_scalars(const GRRTVolTables* t, double* out, int ca
  * `out` if it fits; returns the table length, or -1 for an unknown name. */
 GRRT_EXPORT int64_t grrt_ext_vol_table(const GRRTVolTables* t, const char* name, double* out, int64_t cap);
 GRRT_EXPORT void grrt_ext_vol_free(GRRTVolTables* t);
+/* The scene grrt_create builds from `params` under the reference's
+ * zero-means-default rules (src/api.cpp:43-126), without a GPU.  Values in
+ * this order: width, height, M, a (dimensional), r_obs, theta_obs, phi_obs,
+ * fov, max_steps, tol, thin disk on, volumetric disk on, r_in, r_out,
+ * T_disk, stars, spp, r_horizon, r_isco, then the volumetric disk's own
+ * wiring: its M and spin (the raw params fields), r_outer, T_peak, opacity
+ * nu_min, nu_max.  Returns the count (25), or -1. */
+GRRT_EXPORT int grrt_ext_resolve_params(const GRRTParams* params, double* out, int cap);
 /* Saha ionisation of the H/He/C/O/Fe mixture: n_e, mu, n_ion_eff, n_HI,
  * n_Hminus (CGS). */
 GRRT_EXPORT int grrt_ext_saha(double rho_cgs, double T, double out[5]);
@@ -239,6 +252,22 @@ GRRT_EXPORT int grrt_ext_surrogate_dataset(GRRTContext* ctx, int validation, flo
  * frame in the context's device memory). */
 GRRT_EXPORT int grrt_ext_render_surrogate(GRRTContext* ctx, float* out, int device_out, double entropy_max,
                                           double lng_tol, double judge_fraction, GRRTSurrogateStats* stats);
+/* The surrogate frame's network pass itself, writing the network's 9 raw
+ * outputs per ray to y[n][9] instead of shading: with x the inputs are
+ * x[n][5]; with x NULL they are computed from the frame's sample positions
+ * as the surrogate frame does (n must then be the frame's sample count). */
+GRRT_EXPORT int grrt_ext_surrogate_infer_raw(GRRTContext* ctx, const float* x, int64_t n, float* y);
+/* One Adam step of the training loop's kernel on caller arrays of n floats,
+ * in place on w, m, v (bias corrections 1 - b^t as the training loop forms
+ * them). */
+GRRT_EXPORT int grrt_ext_surrogate_adam_step(GRRTContext* ctx, float* w, const float* g, float* m, float* v,
+                                             int64_t n, double lr, double b1, double b2, double eps, int64_t t);
+/* Sample ids of the last surrogate frame: the first n_fallback rays sent to
+ * the FP64 integrator and the first n_judged judged rays (counts at most
+ * GRRTSurrogateStats fallback_rays / judged_rays; either pointer may be
+ * NULL). */
+GRRT_EXPORT int grrt_ext_surrogate_lists(GRRTContext* ctx, int64_t* fallback, int64_t n_fallback, int64_t* judged,
+                                         int64_t n_judged);
 
 /* ---- Output files --------------------------------------------------------
  * grrt_render_spectral_to_fits_cb (include/grrt/api.h) streams the double

@@ -151,6 +151,8 @@ def lib(variant: str = "strict") -> C.CDLL:
     L.oracle_vol_kappa_es.restype = _D
     L.oracle_simplex.argtypes = [C.POINTER(C.c_int), _D, _D, _D]
     L.oracle_simplex.restype = _D
+    L.oracle_vol_velocity.argtypes = [C.POINTER(OracleVol), _D, _D, _PD]
+    L.oracle_rt_steps.argtypes = [C.c_int, _PD, _PD, _PD, _PD]
     _LIBS[name] = L
     return L
 
@@ -342,6 +344,20 @@ def vol_kappa_abs(vol, nu, rho, T):
 
 def vol_kappa_es(vol, rho, T):
     return lib().oracle_vol_kappa_es(C.byref(vol), rho, T)
+
+
+def vol_velocity(vol, r, theta):
+    u = np.zeros(3)
+    lib().oracle_vol_velocity(C.byref(vol), r, theta, _dp(u))
+    return u
+
+
+def rt_steps(S, dtau, J=0.0, T=1.0):
+    """Backward transfer over emitting samples (S_i, dtau_i); returns (J, T)."""
+    s, d = np.ascontiguousarray(S, np.float64), np.ascontiguousarray(dtau, np.float64)
+    j, t = C.c_double(J), C.c_double(T)
+    lib().oracle_rt_steps(len(s), _dp(s), _dp(d), C.byref(j), C.byref(t))
+    return j.value, t.value
 
 
 def simplex(perm, x, y, z):

@@ -109,6 +109,10 @@ double oracle_vol_temperature(const OracleVol* V, double r, double z);
 double oracle_vol_kappa_abs(const OracleVol* V, double nu, double rho, double T);
 double oracle_vol_kappa_es(const OracleVol* V, double rho, double T);
 double oracle_simplex(const int* perm, double x, double y, double z);
+// Gas 4-velocity (u^t, u^r, u^phi) the ray-march uses at (r, theta).
+void oracle_vol_velocity(const OracleVol* V, double r, double theta, double* u);
+// n emitting samples of the backward transfer on one channel, in order.
+void oracle_rt_steps(int n, const double* S, const double* dtau, double* J, double* T);
 
 #ifdef __cplusplus
 }

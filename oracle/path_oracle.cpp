@@ -942,6 +942,22 @@ double planck_nu(double nu, double T) {
     return (2.0 * kH * nu * nu * nu / (kC * kC)) / (std::exp(x) - 1.0);
 }
 
+// One emitting sample of the backward transfer (src/geodesic_tracer.cpp:
+// 311-315 RGB, :724-728 spectral): J += T S (1 - e^-dtau), T *= e^-dtau.
+inline void rt_step(double& J, double& T, double S, double dtau) {
+    const double ex = std::exp(-dtau);
+    J += T * S * (1.0 - ex);
+    T *= ex;
+}
+
+// Gas 4-velocity at (r, theta) as the ray-march takes it: circular orbit
+// outside the ISCO, plunging geodesic inside (src/volumetric_disk.cpp:95-133).
+inline void gas_velocity(const Vol& VD, double r, double theta, double& ut, double& ur, double& uphi) {
+    ut = ur = uphi = 0.0;
+    if (r >= VD.v.r_isco) VD.circular(r, ut, uphi);
+    else VD.plunging(r, theta, ut, ur, uphi);
+}
+
 // Backward ray-march (src/geodesic_tracer.cpp:183-337 RGB, :605-746
 // spectral).  nb channels at frequencies nu; RGB mode exits on opacity,
 // spectral mode on tau > 10 in every bin; step control on bin `med`.
@@ -990,9 +1006,8 @@ void raymarch(const Scene& S, const Vol& VD, State& state, int nb, const double*
         const double rho = VD.density(r, z, phi);
         const double Tl = VD.temperature(r, std::fabs(z));
         if (rho <= 0.0 || Tl <= 0.0) { state = ns; continue; }
-        double ut = 0.0, ur = 0.0, uphi = 0.0;
-        if (r >= VD.v.r_isco) VD.circular(r, ut, uphi);
-        else VD.plunging(r, theta, ut, ur, uphi);
+        double ut, ur, uphi;
+        gas_velocity(VD, r, theta, ut, ur, uphi);
         const double pue = ns.p[0] * ut + ns.p[1] * ur + ns.p[3] * uphi;
         const double puo = ns.p[0] * ut_obs;
         const double g = pue / puo;
@@ -1007,9 +1022,7 @@ void raymarch(const Scene& S, const Vol& VD, State& state, int nb, const double*
             if (spectral) tau[c] += dtau;
             const double Bnu = planck_nu(nue, Tl);
             const double Sv = eps * Bnu / (nue * nue * nue);
-            const double ex = std::exp(-dtau);
-            J[c] += T[c] * Sv * (1.0 - ex);
-            T[c] *= ex;
+            rt_step(J[c], T[c], Sv, dtau);
         }
         const double alpha_tot = (VD.kappa_abs(std::fabs(g) * nu[med], rho, Tl) + VD.kappa_es(rho, Tl)) * rho;
         const double ds_tau = (alpha_tot > 0.0) ? 0.05 / alpha_tot : ds * 2.0;
@@ -1314,5 +1327,11 @@ double oracle_vol_kappa_abs(const OracleVol* V, double nu, double rho, double T)
 }
 double oracle_vol_kappa_es(const OracleVol* V, double rho, double T) { return Vol{*V}.kappa_es(rho, T); }
 double oracle_simplex(const int* perm, double x, double y, double z) { return Vol::simplex_noise(perm, x, y, z); }
+void oracle_vol_velocity(const OracleVol* V, double r, double theta, double* u) {
+    gas_velocity(Vol{*V}, r, theta, u[0], u[1], u[2]);
+}
+void oracle_rt_steps(int n, const double* S, const double* dtau, double* J, double* T) {
+    for (int i = 0; i < n; ++i) rt_step(*J, *T, S[i], dtau[i]);
+}
 
 }  // extern "C"

@@ -97,7 +97,8 @@ def test_batch_oracle_parity(grrt, oracle):
         same = (c == ocls).reshape(h, w)
         mre = median_rel_err(fb[..., :3][same], ofb[..., :3][same])
         print(f"view spin={spin} {w}x{h}: agreement {agree:.5f} median rel {mre:.2e}", flush=True)
-        assert agree >= 0.999 or not np.any(mism & ~edge)
+        # 99.9% of rays, and at least one ray's slack on the smallest views
+        assert (c != ocls).sum() <= max(1, int(0.001 * w * h)), agree
         assert not np.any(mism & ~edge)
         assert mre <= 1e-6
 
@@ -121,6 +122,70 @@ def test_batch_graph_replay_and_rebuild(grrt):
         ref, _ = _single(grrt, v, stars=True, spp=4, mode=grrt.SAMPLING_HASH)
         diff = np.abs(fb - ref).max(-1) > 1e-9 * np.maximum(np.abs(ref).max(-1), 1e-300)
         assert diff.mean() <= 0.002, diff.mean()
+
+
+def test_uneven_64_view_batch_from_seed_7(grrt, oracle):
+    """The batched-views workload itself: 64 thin-disk Kerr views drawn from
+    seed 7 (a ~ U[0, 0.998], theta ~ U[5, 89] deg, phi ~ U[0, 360) deg, r
+    log-uniform in [20, 100], fov ~ U[30, 90] deg, sides from {128, 256, 512,
+    1024}), pixel centres, in one recorded-graph persistent launch.  Every
+    view: finite frames, one class per ray, and 256 oracle-sampled rays of
+    the view's own scene (class >= 99.9% over all views, median relative HDR
+    error <= 1e-6); replays are bit-identical."""
+    from bench import uneven_views
+    views = uneven_views()
+    assert len(views) == 64
+    print(f"case: 64-view seed-7 batch, {sum(v[5] * v[6] for v in views)} rays", flush=True)
+    with grrt.Context(_base(grrt)) as ctx:
+        ctx.set_sampling(grrt.SAMPLING_CENTER)
+        frames = ctx.render_batch(views)
+        cls = ctx.classes()
+        st = ctx.stats()
+        again = ctx.render_batch(views)
+    assert st.rays == sum(v[5] * v[6] for v in views)
+    assert all(np.array_equal(a, b) for a, b in zip(frames, again))
+    rng = np.random.default_rng(77)
+    o = 0
+    n_agree = n_all = 0
+    errs = []
+    for k, (v, fb) in enumerate(zip(views, frames)):
+        spin, r, th, ph, fov, w, h = v
+        c = cls[o:o + w * h]
+        o += w * h
+        assert fb.shape == (h, w, 4) and np.isfinite(fb).all() and np.all(fb[..., 3] == 1.0)
+        assert np.all(c <= 3)
+        ids = np.sort(rng.choice(w * h, 256, replace=False))
+        sc = oracle.scene(width=w, height=h, mass=1.0, spin_a=spin, r_obs=r, theta_obs=th, phi_obs=ph, fov=fov,
+                          disk=True, r_out=20.0, stars=False)
+        orec = oracle.trace_samples(sc, oracle.JITTER_CENTER, 1, 0, ids)
+        same = c[ids] == orec["cls"]
+        n_agree += int(same.sum())
+        n_all += len(ids)
+        g = fb.reshape(-1, 4)[ids, :3][same].astype(np.float64).ravel()
+        ref = orec["color"][same].ravel()
+        sel = (g != 0) & (ref != 0)
+        errs.append(np.abs(g[sel] - ref[sel]) / np.abs(ref[sel]))
+    e = np.concatenate(errs)
+    print(f"class agreement {n_agree}/{n_all}, median rel HDR {np.median(e):.2e}", flush=True)
+    assert n_agree >= 0.999 * n_all
+    # frames are float32: the oracle's double colour rounds at ~6e-8
+    assert np.median(e) <= 1e-6
+
+
+def test_rejected_batch_keeps_the_recorded_one(grrt):
+    """A larger batch that is rejected (a bad spin after valid views) frees
+    nothing the earlier recording uses: the earlier batch replays to the
+    same frames."""
+    print("case: record a batch, reject a larger one, replay the first", flush=True)
+    with grrt.Context(_base(grrt)) as ctx:
+        ctx.set_sampling(grrt.SAMPLING_CENTER)
+        first = ctx.render_batch(VIEWS[:2])
+        bad = VIEWS + [(0.3, 40.0, 1.0, 0.0, 1.2, 300, 200), (1.5, 50.0, 1.3, 0.0, 1.5, 64, 64)]
+        with pytest.raises(RuntimeError):
+            ctx.render_batch(bad)
+        again = ctx.render_batch(VIEWS[:2])
+        for a, b in zip(first, again):
+            assert np.array_equal(a, b)
 
 
 def test_autotune(grrt):

@@ -128,6 +128,12 @@ def test_kerr_1024_4spp_sampled_parity(grrt, oracle):
     same = g["cls"] == orec["cls"]
     mre = median_rel_err(g["color"][same], orec["color"][same])
     assert mre <= 1e-6, mre
+    # Hamiltonian drift no worse than the oracle's on the sampled rays
+    gh, oh = g["h_max"], orec["h_max"]
+    print(f"|H| max median {np.median(gh):.2e} (oracle {np.median(oh):.2e}), "
+          f"p99 {np.percentile(gh, 99):.2e} (oracle {np.percentile(oh, 99):.2e})", flush=True)
+    assert np.median(gh) <= max(2.0 * np.median(oh), 1e-13)
+    assert np.percentile(gh, 99) <= max(2.0 * np.percentile(oh, 99), 1e-12)
     # size-independent properties of the full frame
     counts = np.bincount(rec["cls"], minlength=4)
     assert counts[3] == 0 and counts[0] > 0 and counts[2] > 0
@@ -160,7 +166,7 @@ def test_edge_cases(grrt, oracle):
     sc = _oracle_scene(oracle, p)
     orec = oracle.trace_samples(sc, oracle.JITTER_CENTER, 1, 0, np.arange(256))
     assert np.mean(rec["cls"] == orec["cls"]) >= 0.98
-    assert np.array_equal(rec["steps"], orec["steps"]) or np.mean(rec["steps"] == orec["steps"]) > 0.9
+    assert np.mean(rec["steps"] == orec["steps"]) >= 0.9
 
 
 @pytest.mark.parametrize("metric,spin", [(0, 0.0), (1, 0.998)])
