@@ -30,6 +30,10 @@ FP64_PEAK_TFLOPS = 78.6           # MI355X FP64 vector peak
 # formulation: 7 Kerr right-hand sides of ~170 flops plus ~450 flops of stage
 # combinations, error norm and control.
 ALGO_FLOPS_PER_ATTEMPT = 1640.0
+# One classic RK4 ray-march step: 4 right-hand sides + ~120 flops, and the
+# transfer of each frequency bin (opacity lookup, source, extinction) ~60.
+ALGO_FLOPS_PER_MARCH_STEP = 800.0
+ALGO_FLOPS_PER_MARCH_BIN = 60.0
 BF16_PEAK_TFLOPS = 2500.0         # MI355X dense BF16 MFMA peak
 # Algorithmic work of one surrogate evaluation: 2 * (5*256 + 3*256*256 + 256*9).
 SURROGATE_FLOPS_PER_RAY = 2.0 * (5 * 256 + 3 * 256 * 256 + 256 * 9)
@@ -50,13 +54,44 @@ def kerr_frame_params(grrt):
                             spp=SPP)
 
 
-def cpu_baseline(target_s: float = 15.0) -> dict:
+def host_cores() -> tuple[int, dict]:
+    """Threads for the CPU baseline: the cores this process may run on
+    (its affinity mask, further limited by OMP_NUM_THREADS when set, which
+    is how the host's share of cores is handed to a job), and a description
+    of the host."""
+    affinity = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(affinity, int(env)) if env.isdigit() and int(env) > 0 else affinity
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return max(1, threads), {"cpu_model": model, "os_cpu_count": os.cpu_count(), "affinity_cores": affinity,
+                             "omp_num_threads_env": env or None}
+
+
+def timed_median(fn, runs: int = 3) -> float:
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[len(ts) // 2]
+
+
+def cpu_baseline(target_s: float = 5.0) -> dict:
     """The CPU oracle built with the reference's release flags, OpenMP over
-    the host cores, on a random sample of the same frame's rays."""
+    the host's cores, on a random sample of the same frame's rays; the
+    median of three timed runs."""
     import numpy as np
     import oracle as O
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads, host = host_cores()
     sc = O.scene(width=W, height=H, spin_a=0.998, r_obs=50.0, theta_obs=math.radians(80.0),
                  fov=math.radians(90.0), disk=True, r_out=20.0, stars=True)
     rng = np.random.default_rng(2024)
@@ -67,13 +102,11 @@ def cpu_baseline(target_s: float = 15.0) -> dict:
     rate = len(probe) / (time.perf_counter() - t0)
     n = int(min(n_total, max(20000, rate * target_s)))
     ids = np.sort(rng.choice(n_total, n, replace=False))
-    t0 = time.perf_counter()
-    O.trace_samples(sc, O.JITTER_SEEDED, SPP, SEED, ids, threads=threads, variant="fast")
-    dt = time.perf_counter() - t0
+    dt = timed_median(lambda: O.trace_samples(sc, O.JITTER_SEEDED, SPP, SEED, ids, threads=threads, variant="fast"))
     return {"value": round(n / dt / 1e6, 5), "unit": "Mrays/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{n} rays drawn uniformly from the same 1024^2x4 frame, {dt:.1f} s, "
-                      f"oracle built -O3 -ffast-math -mavx2 -mfma, OpenMP guided"}
+            "kind": "port", "host": host,
+            "sample": f"{n} rays drawn uniformly from the same 1024^2x4 frame, median of 3 runs of {dt:.1f} s, "
+                      f"oracle built -O3 -ffast-math -mavx2 -mfma -flto, OpenMP guided"}
 
 
 def pixel_flux_error(a, b, sel=None):
@@ -134,6 +167,8 @@ def surrogate_section(grrt, ctx, plain_value, steps, warmup, sg_steps, big_frame
     value = rays / (ms / 1e3) / 1e6
     infer_ms = sum(s["infer_ms"] for s in stats) / steps
     trace_ms = sum(s["trace_ms"] for s in stats) / steps
+    fb_attempts = sum(s["trace_attempts"] for s in stats) / steps
+    fb_achieved = ALGO_FLOPS_PER_ATTEMPT * fb_attempts / (trace_ms / 1e3) / 1e12 if trace_ms > 0 else 0.0
     gpu_ms = sum(s["gpu_ms"] for s in stats) / steps
     achieved = SURROGATE_FLOPS_PER_RAY * rays / (infer_ms / 1e3) / 1e12
     last = stats[-1]
@@ -157,6 +192,10 @@ def surrogate_section(grrt, ctx, plain_value, steps, warmup, sg_steps, big_frame
                      "val_rmse_ln_r": round(ev["rmse_lnr"], 5), "val_rmse_ln_g": round(ev["rmse_lng"], 5)},
         "roofline": {"bound": "mfma", "achieved": round(achieved, 2), "peak": BF16_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / BF16_PEAK_TFLOPS, 4), "traffic": None},
+        # the FP64 fallback/judge trace after the network pass (listed rays)
+        "fallback_roofline": {"bound": "valu", "achieved": round(fb_achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                              "unit": "TFLOP/s", "frac": round(fb_achieved / FP64_PEAK_TFLOPS, 4),
+                              "attempts": int(fb_attempts), "traffic": None},
     }
     if big_frames > 0:
         # The surrogate's own inference frame: Kerr a=0.998, theta=80 deg,
@@ -230,6 +269,7 @@ def batched_views_section(grrt, reps: int) -> dict:
             ctx.render_batch(views, device=True)
         batch_ms = (time.perf_counter() - t0) / reps * 1e3
         kernel_ms = ctx.stats().trace_ms
+        batch_attempts = ctx.stats().attempts
     # The same views one at a time: a context per view (created outside the
     # timed loop), each frame left in device memory like the batch's.
     ctxs = []
@@ -252,11 +292,19 @@ def batched_views_section(grrt, reps: int) -> dict:
                         "{128..1024}), pixel centres, tol 1e-8",
             "rays": rays, "batch_mrays_s": round(rays / batch_ms / 1e3, 3), "batch_ms": round(batch_ms, 3),
             "batch_kernel_ms": round(kernel_ms, 3), "autotuned": best, "autotune_best_ms": round(tune_ms, 3),
+            "roofline": {"bound": "valu",
+                         "achieved": round(ALGO_FLOPS_PER_ATTEMPT * batch_attempts / (kernel_ms / 1e3) / 1e12, 3),
+                         "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(ALGO_FLOPS_PER_ATTEMPT * batch_attempts / (kernel_ms / 1e3) / 1e12
+                                       / FP64_PEAK_TFLOPS, 4), "traffic": None},
             "sequential_mrays_s": round(rays / seq_ms / 1e3, 3), "sequential_ms": round(seq_ms, 3),
             "batch_speedup": round(seq_ms / batch_ms, 3)}
 
 
 CUBE_FREQS = [10 ** (14 + 4 * k / 31) for k in range(32)]
+# The cube's longest ray (pixel x, y): 4159 Dormand-Prince attempts and 3883
+# ray-march steps over 3877 entries into the disk volume (CPU oracle).
+LONGEST_CUBE_RAY = (274, 253)
 
 
 def spectral_cube_params(grrt):
@@ -294,7 +342,7 @@ def spectral_cube_section(grrt, reps: int, cpu: bool) -> dict:
                              "total_host": round(builds["host"][1], 2),
                              "total_device": round(builds["device"][1], 2),
                              "host_threads": max(1, min(16, os.cpu_count() or 1))}
-    for mode, name in ((1, "mixed"), (0, "fp64")):
+    for mode, name in ((1, "mixed"), (0, "fp64"), (2, "mixed_far_field_fp32")):
         ctx.set_precision(mode)
         ctx.render_spectral_f32()
         wall, kern = [], []
@@ -304,9 +352,25 @@ def spectral_cube_section(grrt, reps: int, cpu: bool) -> dict:
             wall.append(time.perf_counter() - t1)
             kern.append(ctx.stats().trace_ms)
         ms = sum(wall) / reps * 1e3
-        out[name] = {"ms_per_cube": round(ms, 2), "trace_kernel_ms": round(sum(kern) / reps, 2),
+        st = ctx.stats()
+        k_ms = sum(kern) / reps
+        flops = (ALGO_FLOPS_PER_ATTEMPT * st.attempts +
+                 (ALGO_FLOPS_PER_MARCH_STEP + ALGO_FLOPS_PER_MARCH_BIN * len(CUBE_FREQS)) * st.march_steps)
+        achieved = flops / (k_ms / 1e3) / 1e12
+        out[name] = {"ms_per_cube": round(ms, 2), "trace_kernel_ms": round(k_ms, 2),
                      "mrays_s": round(rays / (ms / 1e3) / 1e6, 3),
-                     "attempts_per_cube": int(ctx.stats().attempts)}
+                     "attempts_per_cube": int(st.attempts), "march_steps_per_cube": int(st.march_steps),
+                     "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                                  "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None}}
+    # The cube is bound by its longest ray: the same ray traced alone.
+    ctx.set_precision(0)
+    tile = []
+    for _ in range(3):
+        ctx.render_spectral_tile(LONGEST_CUBE_RAY[0], LONGEST_CUBE_RAY[1], 1, 1)
+        tile.append(ctx.stats().trace_ms)
+    out["longest_ray_alone_ms"] = round(sorted(tile)[1], 2)
+    out["longest_ray"] = {"pixel": list(LONGEST_CUBE_RAY), "attempts": int(ctx.stats().attempts),
+                          "march_steps": int(ctx.stats().march_steps)}
     ctx.close()
     if cpu:
         import numpy as np
@@ -380,10 +444,19 @@ def main() -> int:
     avg_trace_s = sum(trace_ms) / len(trace_ms) / 1e3
     att_per_launch = attempts / args.steps
     achieved = ALGO_FLOPS_PER_ATTEMPT * att_per_launch / avg_trace_s / 1e12
-    # host-copy-inclusive rate (grrt_render into a host framebuffer)
-    t1 = time.perf_counter()
-    ctx.render()
-    host_ms = (time.perf_counter() - t1) * 1e3
+    # Rate including the copy to a host framebuffer (grrt_render into the
+    # caller's buffer, reused as a renderer would): median of five after a
+    # warm-up call.
+    import numpy as np
+    host_fb = np.zeros((H, W, 4), np.float32)
+    ctx.render(out=host_fb)
+    host_runs, copy_runs = [], []
+    for _ in range(5):
+        t1 = time.perf_counter()
+        ctx.render(out=host_fb)
+        host_runs.append((time.perf_counter() - t1) * 1e3)
+        copy_runs.append(ctx.stats().copy_ms)
+    host_ms = sorted(host_runs)[2]
     # device tone mapping of the frame (grrt_ext_tonemap_device): 16 B read
     # in the statistics pass, 16 B read and 12 B written in the mapping pass
     tm_ms = []
@@ -416,6 +489,8 @@ def main() -> int:
         "attempts_per_launch": int(att_per_launch),
         "trace_kernel_ms": round(avg_trace_s * 1e3, 3),
         "host_copy_inclusive_ms": round(host_ms, 3),
+        "host_copy_inclusive_mrays_s": round(rays / host_ms / 1e3, 3),
+        "host_copy_ms": round(sorted(copy_runs)[2], 3),
         "tonemap_device": {"gpu_ms": round(tm_ms, 4), "algorithmic_gb_s": round(tm_gbs, 1),
                            "hbm_frac": round(tm_gbs / 8000.0, 4), "bytes_per_pixel": 44},
         "roofline": {"bound": "valu", "achieved": round(achieved, 3),

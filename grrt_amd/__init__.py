@@ -189,6 +189,8 @@ API = [
     ("grrt_ext_get_frame", C.c_int, [_P, _P]),
     ("grrt_ext_set_integrator", C.c_int, [_P, C.c_int]),
     ("grrt_ext_render_spectral_f32", C.c_int, [_P, C.POINTER(C.c_float), C.c_int, C.c_int]),
+    ("grrt_ext_render_spectral_tile", C.c_int, [_P, C.POINTER(C.c_double), C.c_int, C.c_int, C.c_int, C.c_int]),
+    ("grrt_ext_set_vol_tuning", C.c_int, [_P, C.c_int]),
     ("grrt_ext_vol_build", _P, [C.POINTER(GRRTParams)]),
     ("grrt_ext_vol_build_device", _P, [C.POINTER(GRRTParams)]),
     ("grrt_ext_vol_scalars", C.c_int, [_P, C.POINTER(C.c_double), C.c_int]),
@@ -509,8 +511,15 @@ class Context:
         self._check(rc, "grrt_ext_autotune")
         return dict(zip(("tile", "blocks_per_cu", "refill_min", "chunk"), list(best))), ms.value
 
-    def render(self, progress=None) -> np.ndarray:
-        fb = np.zeros((self.params.height, self.params.width, 4), np.float32)
+    def render(self, progress=None, out: np.ndarray | None = None) -> np.ndarray:
+        """grrt_render / grrt_render_cb into a host frame (out, if given, is
+        reused: a (height, width, 4) C-contiguous float32 array)."""
+        if out is None:
+            fb = np.zeros((self.params.height, self.params.width, 4), np.float32)
+        else:
+            assert out.dtype == np.float32 and out.flags.c_contiguous
+            assert out.shape == (self.params.height, self.params.width, 4)
+            fb = out
         ptr = fb.ctypes.data_as(C.POINTER(C.c_float))
         if progress is None:
             rc = self._L.grrt_render(self._h, ptr)
@@ -585,6 +594,10 @@ class Context:
         FP32 far-field geodesic steps (volumetric path)."""
         self._check(self._L.grrt_ext_set_precision(self._h, int(mode)), "set_precision")
 
+    def set_vol_tuning(self, waves_per_simd: int):
+        """Volumetric tracer: 1 or 2 persistent wavefronts per SIMD."""
+        self._check(self._L.grrt_ext_set_vol_tuning(self._h, int(waves_per_simd)), "set_vol_tuning")
+
     def set_integrator(self, mode: int):
         """0: adaptive Dormand-Prince (default); 1: step-doubling RK4."""
         self._check(self._L.grrt_ext_set_integrator(self._h, int(mode)), "set_integrator")
@@ -596,6 +609,13 @@ class Context:
         buf = np.zeros((h, w, max(n, 1)), np.float32)
         self._check(self._L.grrt_ext_render_spectral_f32(
             self._h, buf.ctypes.data_as(C.POINTER(C.c_float)), w, h), "render_spectral_f32")
+        return buf
+
+    def render_spectral_tile(self, x, y, w, h) -> np.ndarray:
+        n = len(getattr(self, "_freqs", []))
+        buf = np.zeros((h, w, max(n, 1)), np.float64)
+        self._check(self._L.grrt_ext_render_spectral_tile(self._h, buf.ctypes.data_as(C.POINTER(C.c_double)),
+                                                           x, y, w, h), "render_spectral_tile")
         return buf
 
     def render_spectral(self, width=None, height=None) -> np.ndarray:

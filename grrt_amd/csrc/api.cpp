@@ -95,6 +95,9 @@ void build_volumetric_tables(GRRTContext* ctx) {
     upload(ctx->vol_T_prof, T.T_profile, ctx->stream);
     upload(ctx->vol_lnk, lnk, ctx->stream);
     upload(ctx->vol_kes, T.op.kappa_es, ctx->stream);
+    upload(ctx->vol_lnk_f, std::vector<float>(lnk.begin(), lnk.end()), ctx->stream);
+    upload(ctx->vol_kes_f, std::vector<float>(T.op.kappa_es.begin(), T.op.kappa_es.end()), ctx->stream);
+    if (T.n_r > kVolMaxR) throw HipFailure("volumetric radial tables larger than the tracer's LDS stage");
     upload(ctx->vol_perm, std::vector<int>(T.perm, T.perm + 512), ctx->stream);
     check(hipStreamSynchronize(ctx->stream), "upload volumetric tables");
     VolDev& V = ctx->vol;
@@ -106,10 +109,16 @@ void build_volumetric_tables(GRRTContext* ctx) {
     V.H = ctx->vol_H.ptr; V.z_max = ctx->vol_zmax.ptr; V.rho_mid = ctx->vol_rho_mid.ptr;
     V.rho_prof = ctx->vol_rho_prof.ptr; V.T_prof = ctx->vol_T_prof.ptr;
     V.ln_kabs = ctx->vol_lnk.ptr; V.kes = ctx->vol_kes.ptr;
+    V.ln_kabs_f = ctx->vol_lnk_f.ptr; V.kes_f = ctx->vol_kes_f.ptr;
     V.n_nu = T.op.n_nu; V.n_rho = T.op.n_rho; V.n_T = T.op.n_T;
     V.lnu0 = T.op.log_nu_min; V.lnu1 = T.op.log_nu_max; V.lrho0 = T.op.log_rho_min;
     V.lrho1 = T.op.log_rho_max; V.lT0 = T.op.log_T_min; V.lT1 = T.op.log_T_max;
     V.perm = ctx->vol_perm.ptr;
+    V.r_scale = (V.n_r - 1) / (V.r_outer - V.r_min);
+    V.inv_noise = 1.0 / V.noise_scale;
+    V.nu_scale = (V.n_nu - 1) / (V.lnu1 - V.lnu0);
+    V.rho_scale_ax = (V.n_rho - 1) / (V.lrho1 - V.lrho0);
+    V.T_scale = (V.n_T - 1) / (V.lT1 - V.lT0);
     ctx->vol_build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
@@ -313,7 +322,8 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
                          ctx->stream), "upload params");
     check(launch_reset(P.counters, P.sample_cls, n_rays, ctx->stream), "reset kernel");
     check(hipEventRecord(ctx->ev[0], ctx->stream), "hipEventRecord");
-    if (vol) check(launch_trace_vol(ctx->params_dev.ptr, n_rays, nb, spectral, ctx->mixed_precision, ctx->stream),
+    if (vol) check(launch_trace_vol(ctx->params_dev.ptr, n_rays, nb, spectral, ctx->mixed_precision, ctx->num_cus,
+                                    ctx->vol_waves_per_simd, ctx->stream),
                    "volumetric trace launch");
     else check(launch_trace_thin(ctx->params_dev.ptr, 0, blocks, threads, ctx->stream, ctx->integrator),
                "trace kernel launch");
@@ -587,6 +597,8 @@ void grrt_destroy(GRRTContext* ctx) {
     ctx->vol_T_prof.release();
     ctx->vol_lnk.release();
     ctx->vol_kes.release();
+    ctx->vol_lnk_f.release();
+    ctx->vol_kes_f.release();
     ctx->vol_perm.release();
     ctx->nu_obs.release();
     ctx->lognu_obs.release();
@@ -859,6 +871,14 @@ int grrt_ext_set_tuning(GRRTContext* ctx, int block, int blocks_per_cu, int refi
     });
 }
 
+int grrt_ext_set_vol_tuning(GRRTContext* ctx, int waves_per_simd) {
+    return guarded(ctx, [&] {
+        if (waves_per_simd < 1 || waves_per_simd > 2) throw HipFailure("waves per SIMD must be 1 or 2");
+        ctx->vol_waves_per_simd = waves_per_simd;
+        return 0;
+    });
+}
+
 int grrt_ext_set_precision(GRRTContext* ctx, int mode) {
     return guarded(ctx, [&] {
         if (mode < 0 || mode > 2) throw HipFailure("precision mode must be 0 (FP64), 1 or 2 (mixed)");
@@ -874,6 +894,27 @@ int grrt_ext_set_integrator(GRRTContext* ctx, int mode) {
         ctx->integrator = mode;
         ctx->batch_views.clear();      // the recorded batch carries its parameter block
         return 0;
+    });
+}
+
+int grrt_ext_render_spectral_tile(GRRTContext* ctx, double* out, int x, int y, int tile_width, int tile_height) {
+    return guarded(ctx, [&] {
+        const auto& r = ctx->res;
+        if (!out || tile_width <= 0 || tile_height <= 0 || x < 0 || y < 0 || x + tile_width > r.W ||
+            y + tile_height > r.H)
+            throw HipFailure("grrt_ext_render_spectral_tile: tile outside the frame");
+        if (ctx->frequency_bins.empty()) throw HipFailure("No frequency bins set -- call grrt_set_frequency_bins first");
+        const size_t n = static_cast<size_t>(tile_width) * tile_height * ctx->frequency_bins.size();
+        if (!r.volumetric) {
+            std::memset(out, 0, n * sizeof(double));
+            return 0;
+        }
+        begin_call(ctx);
+        check(ctx->spec_out.ensure(n * sizeof(double)), "hipMalloc(spectral out)");
+        const int rc = render_window(ctx, x, y, tile_width, tile_height, ctx->spec_out.ptr, nullptr, nullptr,
+                                     Output::kSpectralF64);
+        check(hipMemcpy(out, ctx->spec_out.ptr, n * sizeof(double), hipMemcpyDeviceToHost), "copy spectra");
+        return rc;
     });
 }
 

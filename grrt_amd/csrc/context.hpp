@@ -44,8 +44,9 @@ hipError_t launch_reset(DevCounters* c, uint8_t* cls, long long n_rays, hipStrea
 // two, kVolBinsPerLane bins per lane in registers), or 0 when nbins exceeds
 // 64 * kVolBinsPerLane and the bins live in device scratch, one lane per ray.
 int vol_group_size(int nbins);
+// Persistent one-wavefront blocks: waves_per_simd (1 or 2) per SIMD.
 hipError_t launch_trace_vol(const TraceParams* P, long long n_rays, int nbins, bool spectral, bool fp32_rt,
-                            hipStream_t st);
+                            int num_cus, int waves_per_simd, hipStream_t st);
 hipError_t launch_finalize_spectral(const double* spec, const uint8_t* cls, void* out, long long npix, int nS,
                                     int nb, bool f32, hipStream_t st);
 // mode: 0 one view, 1 batch of views, 2 listed rays of one view.
@@ -86,6 +87,7 @@ struct GRRTContext {
     grrt_amd::VolDev vol{};
     double vol_build_ms = 0.0;
     grrt_amd::DevBuf<double> vol_H, vol_zmax, vol_rho_mid, vol_rho_prof, vol_T_prof, vol_lnk, vol_kes;
+    grrt_amd::DevBuf<float> vol_lnk_f, vol_kes_f;
     grrt_amd::DevBuf<int> vol_perm;
     grrt_amd::DevBuf<double> nu_obs, lognu_obs;
     grrt_amd::DevBuf<unsigned char> rt_scratch;
@@ -131,6 +133,7 @@ struct GRRTContext {
     int tune_refill = 8;
     int tune_chunk = 64;
     int tune_tile = 0;
+    int vol_waves_per_simd = 2;        // volumetric tracer: register budget and persistent waves per SIMD
 
     // Batched views: per-view tables and parameter blocks, and the captured
     // launch sequence, reused while the same views are rendered again.

@@ -15,6 +15,7 @@ constexpr double kSpecTmin = 1000.0, kSpecTmax = 100000.0;
 constexpr int kStarGT = 180, kStarGP = 360;   // src/celestial_sphere.h:123-124
 constexpr int kStarCells = kStarGT * kStarGP;
 constexpr int kVolBinsPerLane = 4;  // spectral bins held in registers per lane of a ray's group
+constexpr int kVolMaxR = 512;       // radial tables staged in LDS by the volumetric tracer (n_r = 500)
 
 // Queue and statistics counters in device memory (reset by a kernel before
 // every launch, so the launch sequence is graph-capturable).
@@ -43,11 +44,18 @@ struct VolDev {
     double rho_scale, noise_scale, turbulence;
     int octaves, n_r, n_z;
     const double *H, *z_max, *rho_mid, *rho_prof, *T_prof;   // n_r, n_r*n_z
-    // Opacity tables: ln(max(kappa_abs, 1e-100)) [nu][rho][T] and kappa_es [rho][T].
+    // Opacity tables: ln(max(kappa_abs, 1e-100)) [nu][rho][T] and kappa_es [rho][T],
+    // in FP64 and rounded to FP32 (the FP32-transfer path reads those: half
+    // the bytes, so the absorption table stays in the L2 cache).
     const double *ln_kabs, *kes;
+    const float *ln_kabs_f, *kes_f;
     int n_nu, n_rho, n_T;
     double lnu0, lnu1, lrho0, lrho1, lT0, lT1;
     const int* perm;                     // 512 entries
+    // Reciprocal scales, so the per-sample lookups multiply instead of
+    // dividing: (n_r - 1)/(r_outer - r_min), 1/noise_scale, and (n - 1)/(hi - lo)
+    // of the three log axes of the opacity tables.
+    double r_scale, inv_noise, nu_scale, rho_scale_ax, T_scale;
 };
 
 // Everything that belongs to one camera view: the metric (spin varies per

@@ -11,11 +11,13 @@
 // runs in the template type RT: double for the all-FP64 path, float for the
 // mixed-precision path, where each bin's intensity is accumulated with
 // Neumaier compensated summation.  On request (far_f32) the mixed path also
-// takes the outer Dormand–Prince steps in FP32 while the ray is outside
-// r = 15 M (far-field segments, before it reaches the disk or after it
-// leaves), with the error tolerance raised to 1e-6, the smallest the FP32
-// error estimate resolves.
+// takes the outer Dormand–Prince steps in FP32 on far-field segments: outside
+// r = 15 M, on the way out, away from the disk's step-clamp zone and not
+// right after a theta turning point, with the error tolerance raised to
+// 1e-6, the smallest the FP32 error estimate resolves.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "device_math.hpp"
 #include "kerr_rhs.hpp"
@@ -45,16 +47,30 @@ struct GState {
     double P0, P3;  // conserved p_t, p_phi
 };
 
+// 0.9 x^(1/5) of the step controller (src/rk4.cpp:113-125, pow in FP64):
+// an FP32 estimate from the hardware log2/exp2, then two Newton steps on
+// y^5 = x in FP64, which converge quadratically from ~1e-7 to rounding.
+// The ray-march samples the volume at positions that follow the outer step
+// lengths, so an FP32-rounded step length (~1e-7 of a step) would move its
+// samples and show in the intensities at ~1e-5.
 __device__ __forceinline__ double step_factor_v(double x) {
     const float lx = __builtin_amdgcn_logf(static_cast<float>(x));
-    return static_cast<double>(0.9f * __builtin_amdgcn_exp2f(0.2f * lx));
+    double y = static_cast<double>(__builtin_amdgcn_exp2f(0.2f * lx));
+#pragma unroll
+    for (int it = 0; it < 2; ++it) {
+        const double y2 = y * y, y4 = y2 * y2;
+        y = __builtin_fma(-(__builtin_fma(y4, y, -x)), rcp1(5.0 * y4), y);
+    }
+    return 0.9 * y;
 }
 
-// One adaptive Dormand–Prince step (src/rk4.cpp:279-328); returns next dl,
-// counts attempts.
-__device__ double adaptive_dp45(const KerrK& K, GState& s, double dl, double tol, int& attempts) {
+// One adaptive Dormand–Prince step (src/rk4.cpp:87-136); returns next dl,
+// counts attempts.  k1 is the right-hand side at s on entry and at the new
+// state on return: the accepted stage k7 is evaluated at y5, which becomes
+// the state (first same as last), so the next step and the near-disk step
+// clamp reuse it instead of evaluating it again.
+__device__ double adaptive_dp45(const KerrK& K, GState& s, Vec5& k1, double dl, double tol, int& attempts) {
     double dtmp;
-    const Vec5 k1 = kerr_rhs(K, s.P0, s.P3, s.y, dtmp);
     for (int retry = 0; retry <= 20; ++retry) {
         const double h = dl;
         Vec5 st, k2, k3, k4, k5, k6, k7, y5, e;
@@ -90,17 +106,20 @@ __device__ double adaptive_dp45(const KerrK& K, GState& s, double dl, double tol
         ++attempts;
         if (retry == 20) {
             s.y = y5;
+            k1 = k7;
             return h;
         }
         if (err <= tol) {
             double f = (err > 1e-30) ? step_factor_v(tol / err) : 5.0;
             f = sclamp(f, 1.0, 5.0);
             s.y = y5;
+            k1 = k7;
             return smin(h * f, 5.0 * smax(__builtin_fabs(y5.v[0]), 1.0));
         }
         dl = h * sclamp(step_factor_v(tol / err), 0.2, 1.0);
         if (dl < 1e-6) {
             s.y = y5;
+            k1 = k7;
             return smin(1e-6, 5.0 * smax(__builtin_fabs(y5.v[0]), 1.0));
         }
     }
@@ -245,11 +264,11 @@ __device__ double adaptive_dp45_f32(const KerrK& K, GState& s, double dl, int& a
     return h;   // not reached
 }
 
-// Classic RK4 step (include/grrt/geodesic/rk4.h:34-52).
-__device__ GState rk4_step(const KerrK& K, const GState& s, double h) {
+// Classic RK4 step (include/grrt/geodesic/rk4.h:34-52) from s, whose
+// right-hand side k1 is known.
+__device__ GState rk4_step_from(const KerrK& K, const GState& s, const Vec5& k1, double h) {
     double dt;
     GState t = s, o = s;
-    const Vec5 k1 = kerr_rhs(K, s.P0, s.P3, s.y, dt);
     for (int i = 0; i < 5; ++i) t.y.v[i] = s.y.v[i] + k1.v[i] * (h * 0.5);
     const Vec5 k2 = kerr_rhs(K, s.P0, s.P3, t.y, dt);
     for (int i = 0; i < 5; ++i) t.y.v[i] = s.y.v[i] + k2.v[i] * (h * 0.5);
@@ -259,6 +278,12 @@ __device__ GState rk4_step(const KerrK& K, const GState& s, double h) {
     for (int i = 0; i < 5; ++i)
         o.y.v[i] = s.y.v[i] + (k1.v[i] + k2.v[i] * 2.0 + k3.v[i] * 2.0 + k4.v[i]) * (h / 6.0);
     return o;
+}
+
+__device__ GState rk4_step(const KerrK& K, const GState& s, double h) {
+    double dt;
+    const Vec5 k1 = kerr_rhs(K, s.P0, s.P3, s.y, dt);
+    return rk4_step_from(K, s, k1, h);
 }
 
 // Per-ray radiative-transfer accumulators: J in units of 2h/c^2 (with
@@ -354,6 +379,9 @@ struct Bins<RT, kBinsGroup> {
     __device__ RT& tau(int j) { return tauv[j]; }
 };
 
+__device__ __forceinline__ double rcp_rt(double x) { return rcp(x); }
+__device__ __forceinline__ float rcp_rt(float x) { return __builtin_amdgcn_rcpf(x); }
+
 // Compensated addition (Neumaier 1974): s + c carries the exact running sum.
 __device__ __forceinline__ void neumaier(float& s, float& c, float y) {
     const float t = s + y;
@@ -372,17 +400,19 @@ struct MarchOut {
 // an opacity exit; spectral mode: step control on bin N/2 and an exit once
 // every bin is past tau = 10.  Each lane updates the bins of its slots.
 template <class RT, int kKind>
-__device__ MarchOut raymarch(const TraceParams& P, const int* perm, const KerrK& K, GState& st,
+__device__ MarchOut raymarch(const TraceParams& P, const VolDev& V, const VolRadial& R, const int* perm,
+                             const double* nu_tab, const double* lnu_tab, const KerrK& K, GState& st,
                              BinsOf<RT, kKind>& b, double lnu_rgb) {
     constexpr bool kSpectral = kKind != kBinsRGB;
-    const VolDev& V = P.vol;
     const int med = kSpectral ? P.nbins / 2 : 1;
     const double ut_obs = 1.0 / sqrt(1.0 - 2.0 / P.view.r_obs);
     MarchOut mo{0, 0};
     double r = st.y.v[0];
-    const double z0 = r * cos(st.y.v[1]);
-    const double H0 = vol_H(V, r);
-    bool been_inside = vol_inside(V, r, z0);
+    double sth, cth;
+    sincos_cephes(st.y.v[1], sth, cth);
+    const double z0 = r * cth;
+    const double H0 = vol_H(V, R, r);
+    bool been_inside = vol_inside(V, R, r, z0);
     double ds = been_inside ? H0 / 16.0 : smin(__builtin_fabs(z0) / 8.0, H0 * 2.0);
     // Exit tests use the bins' running extreme values.
     RT ext = kSpectral ? RT(1e30) : RT(0);
@@ -396,12 +426,13 @@ __device__ MarchOut raymarch(const TraceParams& P, const int* perm, const KerrK&
         ++mo.steps;
         r = ns.y.v[0];
         const double th = ns.y.v[1], ph = ns.y.v[2];
-        const double z = r * cos(th);
+        sincos_cephes(th, sth, cth);
+        const double z = r * cth;
         if (r < V.r_horizon || r > V.r_outer) { st = ns; break; }
         if (b.all(kSpectral ? (ext > RT(10)) : (ext < RT(1e-6)))) { st = ns; break; }
-        const double H = vol_H(V, r);
-        if (!vol_inside(V, r, z)) {
-            const double zm = vol_zmax(V, r);
+        const double H = vol_H(V, R, r);
+        if (!vol_inside(V, R, r, z)) {
+            const double zm = vol_zmax(V, R, r);
             if (been_inside && __builtin_fabs(z) > zm + 3.0 * H) { st = ns; break; }
             if (!been_inside) ds = smax(smin(__builtin_fabs(z) / 8.0, H * 2.0), H / 64.0);
             else ds = sclamp(H / 4.0, H / 64.0, H);
@@ -409,34 +440,34 @@ __device__ MarchOut raymarch(const TraceParams& P, const int* perm, const KerrK&
             continue;
         }
         been_inside = true;
-        const double rho = vol_density(V, perm, r, z, ph);
-        const double Tmp = vol_temperature(V, r, __builtin_fabs(z));
+        const double rho = vol_density(V, R, perm, r, z, ph);
+        const double Tmp = vol_temperature(V, R, r, __builtin_fabs(z));
         if (rho <= 0.0 || Tmp <= 0.0) { st = ns; continue; }
         double ut, ur, up;
         vol_velocity(V, r, th, ut, ur, up);
         const double pue = ns.P0 * ut + ns.y.v[3] * ur + ns.P3 * up;
-        const double g = pue / (ns.P0 * ut_obs);
+        const double g = pue * rcp(ns.P0 * ut_obs);
         const double ag = __builtin_fabs(g);
         const double lag = log10(ag);
         const RT dsp = static_cast<RT>(__builtin_fabs(pue) * __builtin_fabs(ds));
         const OpCell<RT> cell = vol_op_cell<RT>(V, rho, Tmp);
         const RT kes = vol_kes<RT>(V, cell);
-        const RT rhoT = static_cast<RT>(rho), TT = static_cast<RT>(Tmp);
+        const RT rhoT = static_cast<RT>(rho), hkT = static_cast<RT>(vol_hk_over_T(Tmp));
         RT kmed = RT(0);
         ext = kSpectral ? RT(1e30) : RT(0);
 #pragma unroll kVolBinsPerLane
         for (int j = 0; j < b.slots(); ++j) {
             const int c = b.bin(j);
             if (c < 0) continue;
-            const double nu_o = kSpectral ? P.nu_obs[c] : (c == 0 ? 2.997924e10 / 450e-7 : (c == 1 ? 2.997924e10 / 550e-7 : 2.997924e10 / 650e-7));
-            const double lnu_o = kSpectral ? P.lognu_obs[c] : lnu_rgb;   // RGB: slot 0 only
+            const double nu_o = kSpectral ? nu_tab[c] : (c == 0 ? 2.997924e10 / 450e-7 : (c == 1 ? 2.997924e10 / 550e-7 : 2.997924e10 / 650e-7));
+            const double lnu_o = kSpectral ? lnu_tab[c] : lnu_rgb;   // RGB: slot 0 only
             const RT nue = static_cast<RT>(ag * nu_o);
             const RT kabs = vol_kabs<RT>(V, cell, lag + lnu_o);
             if (c == med) kmed = kabs;
             const RT ktot = kabs + kes;
-            const RT eps = (ktot > RT(0)) ? kabs / ktot : RT(1);
+            const RT eps = (ktot > RT(0)) ? kabs * rcp_rt(ktot) : RT(1);
             const RT dtau = ktot * rhoT * dsp;
-            const RT src = eps * vol_source(nue, TT);   // S = eps B_nu(nu_e) / nu_e^3 in units of 2h/c^2
+            const RT src = eps * vol_source(nue, hkT);   // S = eps B_nu(nu_e) / nu_e^3 in units of 2h/c^2
             const RT ex = exp(-dtau);
             RT& Tc = b.T(j);
             neumaier(b.J(j), b.Jc(j), Tc * src * (RT(1) - ex));
@@ -451,7 +482,7 @@ __device__ MarchOut raymarch(const TraceParams& P, const int* perm, const KerrK&
         }
         kmed = b.share(kmed, med);
         const double alpha_tot = static_cast<double>((kmed + kes) * rhoT);
-        const double ds_tau = alpha_tot > 0.0 ? 0.05 / alpha_tot : ds * 2.0;
+        const double ds_tau = alpha_tot > 0.0 ? 0.05 * rcp(alpha_tot) : ds * 2.0;
         const double ds_geo = 0.1 * smax(r - V.r_horizon, 0.5);
         ds = sclamp(smin(ds_tau, ds_geo), H / 64.0, H);
         st = ns;
@@ -474,148 +505,218 @@ __device__ __forceinline__ long long centre_out_ray(const ViewParams& V, long lo
 
 }  // namespace
 
-// Outer Dormand–Prince loop with the near-disk step clamp and volume entry
-// detection, ray-marching on every entry.  One lane per ray, or for
-// kBinsGroup `group` lanes per ray (each with its share of the bins).
-template <class RT, int kKind>
-__global__ __launch_bounds__(256) void trace_vol(const TraceParams* Pg, int group) {
+// Persistent volumetric tracer.  A ray is owned by a group of `group`
+// lanes (one lane for kBinsGlobal); each group claims its rays one at a
+// time from the device-wide queue (the window's rows centre-out) and takes
+// the next one as soon as its ray ends, so a group never waits for the
+// other rays of its wavefront and the longest rays (around the disk image)
+// start first.  Per ray: the outer Dormand–Prince loop with the near-disk
+// step clamp and volume entry detection, ray-marching on every entry
+// (src/geodesic_tracer.cpp:23-181, :495-603).  A cancel request, read with
+// each claim, stops further claims; claimed rays always finish.
+//
+// kWaves: waves per SIMD the register budget is cut for.  Two halve each
+// wave's issue share while another wave runs beside it; one gives every
+// wavefront a SIMD (and up to 512 registers) but hides no latency.
+template <class RT, int kKind, int kWaves>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) void trace_vol(const TraceParams* Pg,
+                                                                                       int group) {
     constexpr bool kSpectral = kKind != kBinsRGB;
+    // LDS: the noise permutation and the radial tables (scale height, z_max,
+    // midplane density), which the step clamp, the entry test and every
+    // ray-march sample read several times.
     __shared__ int s_perm[512];
+    __shared__ double s_rad[3 * kVolMaxR];
+    // spectral bins of the lane-group kernel (at most 64 * kVolBinsPerLane)
+    __shared__ double s_nu[kKind == kBinsGroup ? 2 * 64 * kVolBinsPerLane : 1];
     for (int i = threadIdx.x; i < 512; i += blockDim.x) s_perm[i] = Pg->vol.perm[i];
+    if (kKind == kBinsGroup)
+        for (int i = threadIdx.x; i < Pg->nbins; i += blockDim.x) {
+            s_nu[i] = Pg->nu_obs[i];
+            s_nu[64 * kVolBinsPerLane + i] = Pg->lognu_obs[i];
+        }
+    const double* const nu_tab = kKind == kBinsGroup ? s_nu : Pg->nu_obs;
+    const double* const lnu_tab = kKind == kBinsGroup ? s_nu + 64 * kVolBinsPerLane : Pg->lognu_obs;
+    const int n_r = Pg->vol.n_r;
+    for (int i = threadIdx.x; i < n_r; i += blockDim.x) {
+        s_rad[i] = Pg->vol.H[i];
+        s_rad[kVolMaxR + i] = Pg->vol.z_max[i];
+        s_rad[2 * kVolMaxR + i] = Pg->vol.rho_mid[i];
+    }
     __syncthreads();
     const TraceParams& P = *Pg;
-    const long long t = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
-    const long long q = kKind != kBinsGlobal ? t / group : t;
-    if (q >= P.n_rays) return;
-    // A cancelled render starts no new ray; the ray's samples keep the
-    // untraced class marker and finalize writes its pixel as zero.
-    if (P.cancel && *P.cancel) return;
     const VolDev& V = P.vol;
+    const VolRadial R{s_rad, s_rad + kVolMaxR, s_rad + 2 * kVolMaxR};
     const ViewParams& Vw = P.view;
-    const long long id = centre_out_ray(Vw, q);
     const KerrK K{Vw.M, Vw.a, Vw.a * Vw.a, 2.0 * Vw.M};
-    GState s;
-    SampleId sid;
-    decode_ray(Vw, id, sid);      // row-major order: every id is a sample of the window
-    camera_ray(Vw, sid, s.P0, s.P3, s.y);
-    BinsOf<RT, kKind> b(P, id, group);
-    // RGB: log10 of this lane's channel frequency, once per ray.
-    double lnu_rgb = 0.0;
-    if (!kSpectral && b.bin(0) >= 0) {
-        const int c = b.bin(0);
-        lnu_rgb = log10(c == 0 ? 2.997924e10 / 450e-7 : (c == 1 ? 2.997924e10 / 550e-7 : 2.997924e10 / 650e-7));
-    }
+    const int lane = static_cast<int>(threadIdx.x & 63u);
+    const int G = kKind == kBinsGlobal ? 1 : group;
+    const int leader = lane & ~(G - 1);
+    const unsigned long long n_queue = static_cast<unsigned long long>(P.n_rays);
+    const volatile int* cancel = P.cancel;
+    unsigned long long* const claimed = P.claimed;
     const int nb = kSpectral ? P.nbins : 3;
     const double half_pi = kPi / 2.0;
-    double dl = 0.01 * Vw.r_obs;
-    int term = 2, steps = 0, attempts = 0, entries = 0, emitted = 0, march_steps = 0;
-    for (int step = 0; step < P.max_steps; ++step) {
-        const double r = s.y.v[0];
-        if (r < Vw.r_term) { term = 0; break; }
-        if (r > P.r_escape) { term = 1; break; }
-        const GState prev = s;
-        {
-            const double th = s.y.v[1];
-            const double z = r * cos(th);
-            if (r >= V.r_horizon * 0.9 && r <= V.r_outer * 1.5) {
-                const double H = vol_H(V, r), zm = vol_zmax(V, r);
-                if (__builtin_fabs(z) < zm + 6.0 * H) {
-                    double dt;
-                    const Vec5 d = kerr_rhs(K, s.P0, s.P3, s.y, dt);
-                    const double dz = __builtin_fabs(cos(th) * d.v[0] - r * sin(th) * d.v[1]);
-                    if (dz > 1e-20) dl = smin(dl, smax(0.25 * H / dz, H / 64.0));
+    while (true) {
+        unsigned long long qv = 0;
+        int stop = 0;
+        if (lane == leader) {
+            qv = atomicAdd(&P.counters->queue_head, 1ull);
+            stop = cancel ? *cancel : 0;
+            if (claimed && (qv & 63ull) == 0ull)
+                __hip_atomic_store(claimed, qv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+        qv = __shfl(qv, leader);
+        stop = __shfl(stop, leader);
+        if (qv >= n_queue || stop) break;
+        const long long id = centre_out_ray(Vw, static_cast<long long>(qv));
+        GState s;
+        SampleId sid;
+        decode_ray(Vw, id, sid);      // row-major order: every id is a sample of the window
+        camera_ray(Vw, sid, s.P0, s.P3, s.y);
+        BinsOf<RT, kKind> b(P, id, group);
+        // RGB: log10 of this lane's channel frequency, once per ray.
+        double lnu_rgb = 0.0;
+        if (!kSpectral && b.bin(0) >= 0) {
+            const int c = b.bin(0);
+            lnu_rgb = log10(c == 0 ? 2.997924e10 / 450e-7 : (c == 1 ? 2.997924e10 / 550e-7 : 2.997924e10 / 650e-7));
+        }
+        double dl = 0.01 * Vw.r_obs;
+        int term = 2, steps = 0, attempts = 0, entries = 0, emitted = 0, march_steps = 0;
+        double dtmp;
+        Vec5 k1 = kerr_rhs(K, s.P0, s.P3, s.y, dtmp);   // right-hand side at s
+        bool turned = false;                              // p_theta changed sign in the last step
+        for (int step = 0; step < P.max_steps; ++step) {
+            const double r = s.y.v[0];
+            if (r < Vw.r_term) { term = 0; break; }
+            if (r > P.r_escape) { term = 1; break; }
+            const GState prev = s;
+            bool disk_zone = false;
+            {
+                // near-disk step clamp from dz/dlambda at s (k1 is the
+                // derivative the reference evaluates there)
+                double sth, cth;
+                sincos_cephes(s.y.v[1], sth, cth);
+                const double z = r * cth;
+                if (r >= V.r_horizon * 0.9 && r <= V.r_outer * 1.5) {
+                    const double H = vol_H(V, R, r), zm = vol_zmax(V, R, r);
+                    if (__builtin_fabs(z) < zm + 6.0 * H) {
+                        disk_zone = true;
+                        const double dz = __builtin_fabs(cth * k1.v[0] - r * sth * k1.v[1]);
+                        if (dz > 1e-20) dl = smin(dl, smax(0.25 * H * rcp(dz), H / 64.0));
+                    }
+                }
+            }
+            // FP32 far field: outside r = 15 M on the way out (p_r > 0:
+            // r grows along the traced ray), away from the disk's step-clamp
+            // zone and not right after a theta turning point.  Inbound
+            // segments stay FP64: their end point is where the ray meets the
+            // disk, and the coarse ray-march (dtau ~ 0.05 per sample) turns
+            // an FP32-level shift of it into percent-level intensity changes.
+            const double pth_before = s.y.v[4];
+            if (kFarFieldF32<RT> && P.far_f32 && r > kFarField * Vw.M && s.y.v[3] > 0.0 && !disk_zone && !turned) {
+                dl = adaptive_dp45_f32(K, s, dl, attempts);
+                k1 = kerr_rhs(K, s.P0, s.P3, s.y, dtmp);
+            } else if (P.integrator == 1) {
+                dl = adaptive_rk4_doubling(K, s, dl, P.tol, attempts);
+                k1 = kerr_rhs(K, s.P0, s.P3, s.y, dtmp);
+            } else {
+                dl = adaptive_dp45(K, s, k1, dl, P.tol, attempts);
+            }
+            turned = (pth_before < 0.0) != (s.y.v[4] < 0.0);
+            ++steps;
+            bool opaque = false;
+            if (!kSpectral) {
+                bool mine = true;
+#pragma unroll kVolBinsPerLane
+                for (int j = 0; j < b.slots(); ++j)
+                    if (b.bin(j) >= 0) mine = mine && (b.T(j) < RT(1e-6));
+                opaque = b.all(mine);
+            }
+            if (!opaque) {
+                const double tp = prev.y.v[1], tn = s.y.v[1];
+                const double dp = tp - half_pi, dn = tn - half_pi;
+                const double rn = s.y.v[0], rp = prev.y.v[0];
+                double sn, cn, sp_, cp_;
+                sincos_cephes(tn, sn, cn);
+                sincos_cephes(tp, sp_, cp_);
+                const double zn = rn * cn, zp = rp * cp_;
+                const bool crossed = (dp * dn < 0.0) && __builtin_fabs(dp - dn) > 1e-12;
+                const bool inside_now = vol_inside(V, R, rn, zn);
+                const bool near = (__builtin_fabs(zn) < vol_zmax(V, R, rn) + 2.0 * vol_H(V, R, rn) ||
+                                   __builtin_fabs(zp) < vol_zmax(V, R, rp) + 2.0 * vol_H(V, R, rp)) &&
+                                  rn >= V.r_horizon && rn <= V.r_outer;
+                if (crossed || inside_now || near) {
+                    const double rlo = smin(rp, rn), rhi = smax(rp, rn);
+                    if (rhi >= V.r_horizon && rlo <= V.r_outer && rp >= V.r_horizon * 0.9 && rp <= V.r_outer * 1.5) {
+                        GState entry = prev;
+                        const MarchOut mo = raymarch<RT, kKind>(P, V, R, s_perm, nu_tab, lnu_tab, K, entry, b, lnu_rgb);
+                        emitted |= mo.emitted;
+                        march_steps += mo.steps;
+                        ++entries;
+                        s = entry;
+                        k1 = kerr_rhs(K, s.P0, s.P3, s.y, dtmp);
+                        continue;
+                    }
                 }
             }
         }
-        if (kFarFieldF32<RT> && P.far_f32 && r > kFarField * Vw.M) dl = adaptive_dp45_f32(K, s, dl, attempts);
-        else if (P.integrator == 1) dl = adaptive_rk4_doubling(K, s, dl, P.tol, attempts);
-        else dl = adaptive_dp45(K, s, dl, P.tol, attempts);
-        ++steps;
-        bool opaque = false;
+        // I_obs = J nu_obs^3 (2h/c^2 restores CGS units).
+        constexpr double two_h_c2 = 2.0 * 6.626070e-27 / (2.997924e10 * 2.997924e10);
+        double col[3] = {0.0, 0.0, 0.0};
+        double mine = 0.0;               // RGB: this lane's channel (slot 0 of lanes 0-2)
+#pragma unroll kVolBinsPerLane
+        for (int j = 0; j < b.slots(); ++j) {
+            const int c = b.bin(j);
+            if (c < 0) continue;
+            const double nu_o = kSpectral ? P.nu_obs[c] : (c == 0 ? 2.997924e10 / 450e-7 : (c == 1 ? 2.997924e10 / 550e-7 : 2.997924e10 / 650e-7));
+            const double Jt = static_cast<double>(b.J(j)) + static_cast<double>(b.Jc(j));
+            const double I = Jt * two_h_c2 * nu_o * nu_o * nu_o;
+            if (kSpectral) P.sample_spec[id * nb + c] = I;
+            else mine = I;
+        }
         if (!kSpectral) {
-            bool mine = true;
-#pragma unroll kVolBinsPerLane
-            for (int j = 0; j < b.slots(); ++j)
-                if (b.bin(j) >= 0) mine = mine && (b.T(j) < RT(1e-6));
-            opaque = b.all(mine);
-        }
-        if (!opaque) {
-            const double tp = prev.y.v[1], tn = s.y.v[1];
-            const double dp = tp - half_pi, dn = tn - half_pi;
-            const double rn = s.y.v[0], rp = prev.y.v[0];
-            const double zn = rn * cos(tn), zp = rp * cos(tp);
-            const bool crossed = (dp * dn < 0.0) && __builtin_fabs(dp - dn) > 1e-12;
-            const bool inside_now = vol_inside(V, rn, zn);
-            const bool near = (__builtin_fabs(zn) < vol_zmax(V, rn) + 2.0 * vol_H(V, rn) ||
-                               __builtin_fabs(zp) < vol_zmax(V, rp) + 2.0 * vol_H(V, rp)) &&
-                              rn >= V.r_horizon && rn <= V.r_outer;
-            if (crossed || inside_now || near) {
-                const double rlo = smin(rp, rn), rhi = smax(rp, rn);
-                if (rhi >= V.r_horizon && rlo <= V.r_outer && rp >= V.r_horizon * 0.9 && rp <= V.r_outer * 1.5) {
-                    GState entry = prev;
-                    const MarchOut mo = raymarch<RT, kKind>(P, s_perm, K, entry, b, lnu_rgb);
-                    emitted |= mo.emitted;
-                    march_steps += mo.steps;
-                    ++entries;
-                    s = entry;
-                    continue;
-                }
+            const int c = b.bin(0);
+            if (c >= 0) {
+                // stars: the thin tracer's catalogue search (src/renderer.cpp:56-61)
+                if (term == 1 && P.stars) mine += star_sample(P, s.y.v[1], s.y.v[2]);
+                P.sample_rgb[3 * id + c] = mine;
             }
+            for (int k = 0; k < 3; ++k) col[k] = b.share(mine, k);
+        }
+        if (b.lead()) {                 // per-ray outputs: one lane of the group
+            const int cls = emitted ? GRRT_CLASS_DISK
+                                    : (term == 0 ? GRRT_CLASS_CAPTURED
+                                                 : (term == 1 ? GRRT_CLASS_ESCAPED : GRRT_CLASS_MAXSTEPS));
+            P.sample_cls[id] = static_cast<uint8_t>(cls);
+            if (P.records) {
+                GRRTRayRecord& o = P.records[id];
+                o.color[0] = col[0];
+                o.color[1] = col[1];
+                o.color[2] = col[2];
+                o.x_final[0] = 0.0;
+                o.x_final[1] = s.y.v[0];
+                o.x_final[2] = s.y.v[1];
+                o.x_final[3] = s.y.v[2];
+                o.p_final[0] = s.P0;
+                o.p_final[1] = s.y.v[3];
+                o.p_final[2] = s.y.v[4];
+                o.p_final[3] = s.P3;
+                o.r_hit = o.g_hit = 0.0;
+                o.h_max = o.h_final = 0.0;
+                o.term = term;
+                o.cls = cls;
+                o.steps = steps;
+                o.attempts = attempts;
+                o.n_cross = entries;
+                o.pad = march_steps;
+            }
+            atomicAdd(&P.counters->steps, static_cast<unsigned long long>(steps));
+            atomicAdd(&P.counters->attempts, static_cast<unsigned long long>(attempts));
+            atomicAdd(&P.counters->rays_done, 1ull);
+            atomicAdd(&P.counters->march_steps, static_cast<unsigned long long>(march_steps));
         }
     }
-    // I_obs = J nu_obs^3 (2h/c^2 restores CGS units).
-    constexpr double two_h_c2 = 2.0 * 6.626070e-27 / (2.997924e10 * 2.997924e10);
-    double col[3] = {0.0, 0.0, 0.0};
-    double mine = 0.0;               // RGB: this lane's channel (slot 0 of lanes 0-2)
-#pragma unroll kVolBinsPerLane
-    for (int j = 0; j < b.slots(); ++j) {
-        const int c = b.bin(j);
-        if (c < 0) continue;
-        const double nu_o = kSpectral ? P.nu_obs[c] : (c == 0 ? 2.997924e10 / 450e-7 : (c == 1 ? 2.997924e10 / 550e-7 : 2.997924e10 / 650e-7));
-        const double Jt = static_cast<double>(b.J(j)) + static_cast<double>(b.Jc(j));
-        const double I = Jt * two_h_c2 * nu_o * nu_o * nu_o;
-        if (kSpectral) P.sample_spec[id * nb + c] = I;
-        else mine = I;
-    }
-    if (!kSpectral) {
-        const int c = b.bin(0);
-        if (c >= 0) {
-            // stars: the thin tracer's catalogue search (src/renderer.cpp:56-61)
-            if (term == 1 && P.stars) mine += star_sample(P, s.y.v[1], s.y.v[2]);
-            P.sample_rgb[3 * id + c] = mine;
-        }
-        for (int k = 0; k < 3; ++k) col[k] = b.share(mine, k);
-    }
-    if (!b.lead()) return;          // per-ray outputs: one lane of the group
-    const int cls = emitted ? GRRT_CLASS_DISK
-                            : (term == 0 ? GRRT_CLASS_CAPTURED : (term == 1 ? GRRT_CLASS_ESCAPED : GRRT_CLASS_MAXSTEPS));
-    P.sample_cls[id] = static_cast<uint8_t>(cls);
-    if (P.records) {
-        GRRTRayRecord& o = P.records[id];
-        o.color[0] = col[0];
-        o.color[1] = col[1];
-        o.color[2] = col[2];
-        o.x_final[0] = 0.0;
-        o.x_final[1] = s.y.v[0];
-        o.x_final[2] = s.y.v[1];
-        o.x_final[3] = s.y.v[2];
-        o.p_final[0] = s.P0;
-        o.p_final[1] = s.y.v[3];
-        o.p_final[2] = s.y.v[4];
-        o.p_final[3] = s.P3;
-        o.r_hit = o.g_hit = 0.0;
-        o.h_max = o.h_final = 0.0;
-        o.term = term;
-        o.cls = cls;
-        o.steps = steps;
-        o.attempts = attempts;
-        o.n_cross = entries;
-        o.pad = march_steps;
-    }
-    atomicAdd(&P.counters->steps, static_cast<unsigned long long>(steps));
-    atomicAdd(&P.counters->attempts, static_cast<unsigned long long>(attempts));
-    atomicAdd(&P.counters->rays_done, 1ull);
-    atomicAdd(&P.counters->march_steps, static_cast<unsigned long long>(march_steps));
 }
 
 // Average the N-bin spectra of each pixel (src/renderer.cpp:235-266); the
@@ -642,27 +743,33 @@ int vol_group_size(int nbins) {
     return g;
 }
 
-hipError_t launch_trace_vol(const TraceParams* P, long long n_rays, int nbins, bool spectral, bool fp32_rt,
-                            hipStream_t st) {
-    const int G = spectral ? vol_group_size(nbins) : kRgbGroup;
-    // Lane groups: one wavefront per block, so a block is dispatched as
-    // soon as one wave slot frees and the long-running rays of the disk
-    // region do not hold back whole 4-wave blocks (the 512^2 x 32-bin cube:
-    // 42 ms against 89 ms with 256 threads).  One lane per ray (more than
-    // 256 bins): 256 threads.
-    const int T = G > 0 ? 64 : 256;
-    const long long threads = n_rays * (G > 0 ? G : 1);
-    const dim3 grid(static_cast<unsigned>((threads + T - 1) / T));
+template <int kWaves>
+void launch_vol_kernel(const TraceParams* P, bool spectral, bool fp32_rt, int G, dim3 grid, hipStream_t st) {
+    const dim3 blk(64);
     if (spectral && G > 0) {
-        if (fp32_rt) hipLaunchKernelGGL((trace_vol<float, kBinsGroup>), grid, dim3(T), 0, st, P, G);
-        else hipLaunchKernelGGL((trace_vol<double, kBinsGroup>), grid, dim3(T), 0, st, P, G);
+        if (fp32_rt) hipLaunchKernelGGL((trace_vol<float, kBinsGroup, kWaves>), grid, blk, 0, st, P, G);
+        else hipLaunchKernelGGL((trace_vol<double, kBinsGroup, kWaves>), grid, blk, 0, st, P, G);
     } else if (spectral) {
-        if (fp32_rt) hipLaunchKernelGGL((trace_vol<float, kBinsGlobal>), grid, dim3(T), 0, st, P, 1);
-        else hipLaunchKernelGGL((trace_vol<double, kBinsGlobal>), grid, dim3(T), 0, st, P, 1);
+        if (fp32_rt) hipLaunchKernelGGL((trace_vol<float, kBinsGlobal, kWaves>), grid, blk, 0, st, P, 1);
+        else hipLaunchKernelGGL((trace_vol<double, kBinsGlobal, kWaves>), grid, blk, 0, st, P, 1);
     } else {
-        if (fp32_rt) hipLaunchKernelGGL((trace_vol<float, kBinsRGB>), grid, dim3(T), 0, st, P, G);
-        else hipLaunchKernelGGL((trace_vol<double, kBinsRGB>), grid, dim3(T), 0, st, P, G);
+        if (fp32_rt) hipLaunchKernelGGL((trace_vol<float, kBinsRGB, kWaves>), grid, blk, 0, st, P, G);
+        else hipLaunchKernelGGL((trace_vol<double, kBinsRGB, kWaves>), grid, blk, 0, st, P, G);
     }
+}
+
+hipError_t launch_trace_vol(const TraceParams* P, long long n_rays, int nbins, bool spectral, bool fp32_rt,
+                            int num_cus, int waves_per_simd, hipStream_t st) {
+    const int G = spectral ? vol_group_size(nbins) : kRgbGroup;
+    // Persistent one-wavefront blocks, one per wave slot the register
+    // budget leaves (4 SIMDs per CU), never more than the queue can feed.
+    const long long per_wave = 64 / (G > 0 ? G : 1);
+    const long long need = (n_rays + per_wave - 1) / per_wave;
+    const int waves = waves_per_simd >= 2 ? 2 : 1;
+    const long long blocks = std::max<long long>(1, std::min<long long>(need, 4LL * num_cus * waves));
+    const dim3 grid(static_cast<unsigned>(blocks));
+    if (waves == 2) launch_vol_kernel<2>(P, spectral, fp32_rt, G, grid, st);
+    else launch_vol_kernel<1>(P, spectral, fp32_rt, G, grid, st);
     return hipGetLastError();
 }
 

@@ -18,28 +18,34 @@ namespace vcgs {
 constexpr double h = 6.626070e-27, k_B = 1.380649e-16, c = 2.997924e10;
 }
 
+// The radial tables (scale height, z_max, midplane density), read from
+// wherever the caller staged them (LDS in the tracer).
+struct VolRadial {
+    const double *H, *z_max, *rho_mid;
+};
+
 __device__ __forceinline__ double vol_interp_radial(const VolDev& V, const double* lut, double r) {
     if (r <= V.r_min) return lut[0];
     if (r >= V.r_outer) return lut[V.n_r - 1];
-    const double frac = (r - V.r_min) / (V.r_outer - V.r_min) * (V.n_r - 1);
+    const double frac = (r - V.r_min) * V.r_scale;
     const int idx = clamp_index(static_cast<int>(frac), V.n_r - 2);
     const double t = frac - idx;
     return lut[idx] * (1.0 - t) + lut[idx + 1] * t;
 }
 
 // Per-column z-normalised 2-D lookup (src/volumetric_disk.cpp:175-214).
-__device__ double vol_interp_2d(const VolDev& V, const double* lut, double r, double za) {
-    double rf = (r - V.r_min) / (V.r_outer - V.r_min) * (V.n_r - 1);
+__device__ double vol_interp_2d(const VolDev& V, const VolRadial& R, const double* lut, double r, double za) {
+    double rf = (r - V.r_min) * V.r_scale;
     rf = rf < 0.0 ? 0.0 : (rf > V.n_r - 1 ? static_cast<double>(V.n_r - 1) : rf);
     const int ri = clamp_index(static_cast<int>(rf), V.n_r - 2);
     const double tr = rf - ri;
     const int ri1 = ri + 1 < V.n_r - 1 ? ri + 1 : V.n_r - 1;
-    const double zlo = V.z_max[ri], zhi = V.z_max[ri1];
+    const double zlo = R.z_max[ri], zhi = R.z_max[ri1];
     if ((zlo <= 0.0 || za >= zlo) && (zhi <= 0.0 || za >= zhi)) return 0.0;
     const int nz = V.n_z;
     auto column = [&](int rr, double zm) -> double {
         if (!(zm > 0.0 && za < zm)) return 0.0;
-        double zf = za / zm * (nz - 1);
+        double zf = za * rcp(zm) * (nz - 1);
         zf = zf < 0.0 ? 0.0 : (zf > nz - 1 ? static_cast<double>(nz - 1) : zf);
         const int zi = clamp_index(static_cast<int>(zf), nz - 2);
         const double tz = zf - zi;
@@ -48,12 +54,16 @@ __device__ double vol_interp_2d(const VolDev& V, const double* lut, double r, do
     return (1.0 - tr) * column(ri, zlo) + tr * column(ri1, zhi);
 }
 
-__device__ __forceinline__ double vol_H(const VolDev& V, double r) { return vol_interp_radial(V, V.H, r); }
-__device__ __forceinline__ double vol_zmax(const VolDev& V, double r) { return vol_interp_radial(V, V.z_max, r); }
+__device__ __forceinline__ double vol_H(const VolDev& V, const VolRadial& R, double r) {
+    return vol_interp_radial(V, R.H, r);
+}
+__device__ __forceinline__ double vol_zmax(const VolDev& V, const VolRadial& R, double r) {
+    return vol_interp_radial(V, R.z_max, r);
+}
 
-__device__ __forceinline__ bool vol_inside(const VolDev& V, double r, double z) {
+__device__ __forceinline__ bool vol_inside(const VolDev& V, const VolRadial& R, double r, double z) {
     if (r <= V.r_horizon || r > V.r_outer) return false;
-    return __builtin_fabs(z) < vol_zmax(V, r) + 1.5 * vol_H(V, r);
+    return __builtin_fabs(z) < vol_zmax(V, R, r) + 1.5 * vol_H(V, R, r);
 }
 
 __device__ __forceinline__ double vol_taper(const VolDev& V, double r) {
@@ -113,10 +123,10 @@ __device__ double fbm3(const int* perm, double x, double y, double z, int octave
 }
 
 // Density [g/cm^3] with photosphere soft edge, ISCO taper and fBm turbulence.
-__device__ double vol_density(const VolDev& V, const int* perm, double r, double z, double phi) {
+__device__ double vol_density(const VolDev& V, const VolRadial& R, const int* perm, double r, double z, double phi) {
     if (r <= V.r_horizon || r > V.r_outer) return 0.0;
     const double za = __builtin_fabs(z);
-    const double zm = vol_zmax(V, r), H = vol_H(V, r);
+    const double zm = vol_zmax(V, R, r), H = vol_H(V, R, r);
     double edge = 1.0, zl = za;
     if (za >= zm) {
         const double dz = za - zm;
@@ -124,49 +134,53 @@ __device__ double vol_density(const VolDev& V, const int* perm, double r, double
         edge = exp(-2.0 * (dz / H) * (dz / H));
         zl = zm * 0.9999;
     }
-    const double base = vol_interp_radial(V, V.rho_mid, r) * vol_interp_2d(V, V.rho_prof, r, zl) *
+    const double base = vol_interp_radial(V, R.rho_mid, r) * vol_interp_2d(V, R, V.rho_prof, r, zl) *
                         V.rho_scale * vol_taper(V, r) * edge;
     double sp, cp;
-    sincos(phi, &sp, &cp);
-    const double n = fbm3(perm, r * cp / V.noise_scale, r * sp / V.noise_scale, z / V.noise_scale, V.octaves);
+    sincos_cephes(phi, sp, cp);
+    const double n = fbm3(perm, r * cp * V.inv_noise, r * sp * V.inv_noise, z * V.inv_noise, V.octaves);
     return smax(0.0, base * (1.0 + V.turbulence * n));
 }
 
-__device__ double vol_temperature(const VolDev& V, double r, double za) {
+__device__ double vol_temperature(const VolDev& V, const VolRadial& R, double r, double za) {
     if (r <= V.r_horizon || r > V.r_outer) return 0.0;
-    const double zm = vol_zmax(V, r), H = vol_H(V, r);
+    const double zm = vol_zmax(V, R, r), H = vol_H(V, R, r);
     if (za >= zm + 1.5 * H) return 0.0;
-    return vol_interp_2d(V, V.T_prof, r, smin(za, zm * 0.9999));
+    return vol_interp_2d(V, R, V.T_prof, r, smin(za, zm * 0.9999));
 }
 
 // Emitter 4-velocity: circular prograde orbit outside the ISCO, plunging
 // with the ISCO energy and angular momentum inside.
 __device__ void vol_velocity(const VolDev& V, double r, double theta, double& ut, double& ur, double& up) {
     const double M = V.M, a = V.a;
+    const double ir = rcp(r);                    // the divisions by r as products
     if (r >= V.r_isco) {
-        const double Om = sqrt(M) / (r * sqrt(r) + a * sqrt(M));
-        const double gtt = -(1.0 - 2.0 * M / r), gtp = -2.0 * M * a / r, gpp = r * r + a * a + 2.0 * M * a * a / r;
+        const double sM = sqrt(M);
+        const double Om = sM * rcp(r * sqrt(r) + a * sM);
+        const double gtt = -(1.0 - 2.0 * M * ir), gtp = -2.0 * M * a * ir, gpp = r * r + a * a + 2.0 * M * a * a * ir;
         const double den = -(gtt + 2.0 * gtp * Om + gpp * Om * Om);
-        ut = 1.0 / sqrt(smax(den, 1e-30));
+        ut = rcp(sqrt(smax(den, 1e-30)));
         ur = 0.0;
         up = Om * ut;
         return;
     }
     const double E = V.E_isco, L = V.L_isco;
     const double Del = r * r - 2.0 * M * r + a * a;
-    const double ct = cos(theta);
+    const double iDel = rcp(Del);
+    double st, ct;
+    sincos_cephes(theta, st, ct);
     const double Sig = r * r + a * a * ct * ct;
-    ut = (E * (r * r + a * a + 2.0 * M * a * a / r) - 2.0 * M * a * L / r) / Del;
-    up = (L * (1.0 - 2.0 * M / r) + 2.0 * M * a * E / r) / Del;
+    ut = (E * (r * r + a * a + 2.0 * M * a * a * ir) - 2.0 * M * a * L * ir) * iDel;
+    up = (L * (1.0 - 2.0 * M * ir) + 2.0 * M * a * E * ir) * iDel;
     const double EaL = E * (r * r + a * a) - a * L, LaE = L - a * E;
     const double Rr = EaL * EaL - Del * (r * r + LaE * LaE);
-    ur = -sqrt(smax(0.0, Rr)) / Sig;
+    ur = -sqrt(smax(0.0, Rr)) * rcp(Sig);
 }
 
 // Cell index and fraction on a log-spaced axis, clamped to the table.
 template <class T>
-__device__ __forceinline__ void vol_log_cell(T lv, T lo, T hi, int n, int& idx, T& frac) {
-    T t = (lv - lo) / (hi - lo) * static_cast<T>(n - 1);
+__device__ __forceinline__ void vol_log_cell(T lv, T lo, T scale, int n, int& idx, T& frac) {
+    T t = (lv - lo) * scale;     // scale = (n - 1) / (hi - lo)
     t = t < T(0) ? T(0) : (t > static_cast<T>(n - 2) ? static_cast<T>(n - 2) : t);
     idx = clamp_index(static_cast<int>(t), n - 2);
     frac = t - static_cast<T>(idx);
@@ -182,16 +196,23 @@ struct OpCell {
 template <class T>
 __device__ __forceinline__ OpCell<T> vol_op_cell(const VolDev& V, double rho, double Temp) {
     OpCell<T> c;
-    vol_log_cell<T>(static_cast<T>(log10(rho)), static_cast<T>(V.lrho0), static_cast<T>(V.lrho1), V.n_rho, c.irho, c.frho);
-    vol_log_cell<T>(static_cast<T>(log10(Temp)), static_cast<T>(V.lT0), static_cast<T>(V.lT1), V.n_T, c.iT, c.fT);
+    vol_log_cell<T>(static_cast<T>(log10(rho)), static_cast<T>(V.lrho0), static_cast<T>(V.rho_scale_ax), V.n_rho, c.irho,
+                    c.frho);
+    vol_log_cell<T>(static_cast<T>(log10(Temp)), static_cast<T>(V.lT0), static_cast<T>(V.T_scale), V.n_T, c.iT, c.fT);
     return c;
 }
 
 // kappa_es bilinear in (log rho, log T) cell coordinates (src/opacity.cpp:339-346).
+// The opacity tables in the transfer's own precision.
+__device__ __forceinline__ const double* vol_kes_table(const VolDev& V, double) { return V.kes; }
+__device__ __forceinline__ const float* vol_kes_table(const VolDev& V, float) { return V.kes_f; }
+__device__ __forceinline__ const double* vol_lnk_table(const VolDev& V, double) { return V.ln_kabs; }
+__device__ __forceinline__ const float* vol_lnk_table(const VolDev& V, float) { return V.ln_kabs_f; }
+
 template <class T>
 __device__ __forceinline__ T vol_kes(const VolDev& V, const OpCell<T>& c) {
     const int n = V.n_T;
-    const double* q = V.kes + c.irho * n + c.iT;
+    const auto* q = vol_kes_table(V, T(0)) + c.irho * n + c.iT;
     return static_cast<T>(q[0]) * (T(1) - c.frho) * (T(1) - c.fT) + static_cast<T>(q[1]) * (T(1) - c.frho) * c.fT +
            static_cast<T>(q[n]) * c.frho * (T(1) - c.fT) + static_cast<T>(q[n + 1]) * c.frho * c.fT;
 }
@@ -201,9 +222,9 @@ template <class T>
 __device__ __forceinline__ T vol_kabs(const VolDev& V, const OpCell<T>& c, double lognu) {
     int inu;
     T fnu;
-    vol_log_cell<T>(static_cast<T>(lognu), static_cast<T>(V.lnu0), static_cast<T>(V.lnu1), V.n_nu, inu, fnu);
+    vol_log_cell<T>(static_cast<T>(lognu), static_cast<T>(V.lnu0), static_cast<T>(V.nu_scale), V.n_nu, inu, fnu);
     const int nrt = V.n_rho * V.n_T;
-    const double* q = V.ln_kabs + inu * nrt + c.irho * V.n_T + c.iT;
+    const auto* q = vol_lnk_table(V, T(0)) + inu * nrt + c.irho * V.n_T + c.iT;
     const int n = V.n_T;
     const T l00 = static_cast<T>(q[0]) * (T(1) - c.fT) + static_cast<T>(q[1]) * c.fT;
     const T l01 = static_cast<T>(q[n]) * (T(1) - c.fT) + static_cast<T>(q[n + 1]) * c.fT;
@@ -217,17 +238,19 @@ __device__ __forceinline__ T vol_kabs(const VolDev& V, const OpCell<T>& c, doubl
 // Invariant source B_nu / nu^3 in units of 2h/c^2: 1/(exp(h nu/kT) - 1),
 // zero for h nu/kT > 500 (src/opacity.cpp:209-214 divided by nu^3).  The
 // scaling keeps the FP32 path clear of underflow (B_nu/nu^3 ~ 1e-47 cgs).
-__device__ __forceinline__ double vol_source(double nu, double Temp) {
-    if (nu <= 0.0 || Temp <= 0.0) return 0.0;
-    const double x = (vcgs::h / vcgs::k_B) * nu / Temp;
+// hk_T = h / (k_B T) of the sample, formed once for all its bins.
+__device__ __forceinline__ double vol_source(double nu, double hk_T) {
+    if (nu <= 0.0 || hk_T <= 0.0) return 0.0;
+    const double x = hk_T * nu;
     if (x > 500.0) return 0.0;
-    return 1.0 / (exp(x) - 1.0);
+    return rcp(exp(x) - 1.0);
 }
-__device__ __forceinline__ float vol_source(float nu, float Temp) {
-    if (nu <= 0.0f || Temp <= 0.0f) return 0.0f;
-    const float x = static_cast<float>(vcgs::h / vcgs::k_B) * nu / Temp;
+__device__ __forceinline__ float vol_source(float nu, float hk_T) {
+    if (nu <= 0.0f || hk_T <= 0.0f) return 0.0f;
+    const float x = hk_T * nu;
     if (x > 500.0f) return 0.0f;
-    return 1.0f / expm1f(x);
+    return __builtin_amdgcn_rcpf(expm1f(x));
 }
+__device__ __forceinline__ double vol_hk_over_T(double Temp) { return Temp > 0.0 ? (vcgs::h / vcgs::k_B) / Temp : 0.0; }
 
 }  // namespace grrt_amd

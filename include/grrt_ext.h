@@ -143,6 +143,14 @@ GRRT_EXPORT int grrt_ext_set_precision(GRRTContext* ctx, int mode);
  * tracers. */
 GRRT_EXPORT int grrt_ext_set_integrator(GRRTContext* ctx, int mode);
 /* grrt_render_spectral into a float32 cube (width*height*nbins). */
+/* The spectra of the pixel window [x, x+w) x [y, y+h) of the context's
+ * frame (w*h*bins doubles, the same values as those pixels of
+ * grrt_render_spectral): the spectral counterpart of grrt_render_tile. */
+GRRT_EXPORT int grrt_ext_render_spectral_tile(GRRTContext* ctx, double* out, int x, int y, int tile_width,
+                                              int tile_height);
+/* Launch shape of the volumetric tracer: 1 or 2 persistent wavefronts per
+ * SIMD (the kernel's register budget is cut to match). */
+GRRT_EXPORT int grrt_ext_set_vol_tuning(GRRTContext* ctx, int waves_per_simd);
 GRRT_EXPORT int grrt_ext_render_spectral_f32(GRRTContext* ctx, float* out, int width, int height);
 
 /* Access to the volumetric disk set-up (src/volumetric_disk.cpp:33-73,

@@ -102,7 +102,10 @@ def build() -> None:
 
 
 def lib(variant: str = "strict") -> C.CDLL:
-    name = "liboracle.so" if variant == "strict" else "liboracle_fast.so"
+    """strict: the checker; fast: the reference's release flags (self-noise
+    and the CPU baseline); ulp: strict with one-ulp noise in the Kerr
+    right-hand side (the rounding-noise floor of each output)."""
+    name = {"strict": "liboracle.so", "fast": "liboracle_fast.so", "ulp": "liboracle_ulp.so"}[variant]
     if name in _LIBS:
         return _LIBS[name]
     path = os.path.join(_HERE, "lib", name)

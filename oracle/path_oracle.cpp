@@ -123,6 +123,25 @@ inline void kerr_derivs(double M, double a, const State& s, State& d) {
     d.p[1] = dp_r;
     d.p[2] = dp_t;
     d.p[3] = 0.0;
+#ifdef ORACLE_RHS_ULP_NOISE
+    // Rounding-noise variant (lib/liboracle_ulp.so): every evolving
+    // component of the right-hand side moved by one ulp up or down (a hash of
+    // its bits picks the direction).  Compared with the strict build it
+    // measures how far rounding-level differences in the geodesic carry into
+    // each output, the floor for any other arithmetic of the same path.
+    auto ulp_step = [](double v) {
+        uint64_t b;
+        std::memcpy(&b, &v, sizeof b);
+        uint64_t h = b * 0x9E3779B97F4A7C15ull;
+        h ^= h >> 29;
+        return (h & 1) ? std::nextafter(v, 1e300) : std::nextafter(v, -1e300);
+    };
+    d.x[1] = ulp_step(d.x[1]);
+    d.x[2] = ulp_step(d.x[2]);
+    d.x[3] = ulp_step(d.x[3]);
+    d.p[1] = ulp_step(d.p[1]);
+    d.p[2] = ulp_step(d.p[2]);
+#endif
 }
 
 // src/kerr.cpp:19-40 — covariant metric with libm sin/cos and pole guards.
@@ -961,8 +980,9 @@ inline void gas_velocity(const Vol& VD, double r, double theta, double& ut, doub
 // Backward ray-march (src/geodesic_tracer.cpp:183-337 RGB, :605-746
 // spectral).  nb channels at frequencies nu; RGB mode exits on opacity,
 // spectral mode on tau > 10 in every bin; step control on bin `med`.
-void raymarch(const Scene& S, const Vol& VD, State& state, int nb, const double* nu, bool spectral,
-              double* J, double* T, double* tau, bool& emitted) {
+// Returns the RK4 steps taken.
+int raymarch(const Scene& S, const Vol& VD, State& state, int nb, const double* nu, bool spectral,
+             double* J, double* T, double* tau, bool& emitted) {
     const double M = S.p.mass, a = S.p.spin_a;
     const double ut_obs = 1.0 / std::sqrt(1.0 - 2.0 / S.p.r_obs);
     const int med = spectral ? nb / 2 : 1;
@@ -1032,6 +1052,7 @@ void raymarch(const Scene& S, const Vol& VD, State& state, int nb, const double*
         state = ns;
         emitted = true;
     }
+    return step_count;
 }
 
 // Outer loop with a volumetric disk (src/geodesic_tracer.cpp:23-181 RGB,
@@ -1050,6 +1071,7 @@ void vol_trace(const Scene& S, const Vol& VD, const State& init, int nbins, cons
     double dl = 0.01 * S.p.r_obs;
     int term = 2;
     bool emitted = false;
+    int march = 0;
     o.steps = o.attempts = o.n_cross = 0;
     for (int step = 0; step < S.p.max_steps; ++step) {
         const double r = st.x[1];
@@ -1090,7 +1112,7 @@ void vol_trace(const Scene& S, const Vol& VD, const State& init, int nbins, cons
                     State entry = prev;
                     const double re = entry.x[1];
                     if (re >= VD.v.r_horizon * 0.9 && re <= VD.v.r_outer * 1.5) {
-                        raymarch(S, VD, entry, nb, nu, spectral, J.data(), T.data(), tau.data(), emitted);
+                        march += raymarch(S, VD, entry, nb, nu, spectral, J.data(), T.data(), tau.data(), emitted);
                         ++o.n_cross;
                         st = entry;
                         continue;
@@ -1114,7 +1136,7 @@ void vol_trace(const Scene& S, const Vol& VD, const State& init, int nbins, cons
     o.term = term;
     o.cls = emitted ? 2 : (term == 0 ? 0 : (term == 1 ? 1 : 3));
     o.r_hit = o.g_hit = o.h_max = o.h_final = 0.0;
-    o.pad = 0;
+    o.pad = march;                   // ray-march RK4 steps over all entries
 }
 
 }  // namespace

@@ -204,7 +204,14 @@ def test_adam_kernel_matches_oracle(grrt, oracle):
     rel = np.abs(du_g - du_o) / np.maximum(np.abs(du_o), 1e-7)
     print(f"w update: median rel {np.median(rel):.2e} max {rel.max():.2e}", flush=True)
     assert np.median(rel) <= 1e-5
-    assert np.abs(wg.astype(np.float64) - wo).max() <= 4 * np.spacing(np.abs(wo)).max()
+    # every element: the FP32 update to ~1e-5 of itself, plus the rounding of w
+    ulp = np.spacing(np.abs(wo)).astype(np.float64)
+    # (three steps: three roundings of w, plus the moments' contraction)
+    viol = np.abs(du_g - du_o) / (1e-5 * np.abs(du_o) + ulp)
+    for i in np.argsort(-viol)[:3]:
+        print(f"worst {i}: ratio {viol[i]:.2f} du_gpu {du_g[i]:.6e} du_oracle {du_o[i]:.6e} w0 {w0[i]:.3e} "
+              f"m0 {m0[i]:.3e} v0 {v0[i]:.3e} m {mo[i]:.3e} v {vo[i]:.3e}", flush=True)
+    assert viol.max() <= 4.0
 
 
 def _judge_from_plain(ids, scls, pcls, srgb, prgb):

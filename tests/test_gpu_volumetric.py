@@ -5,9 +5,18 @@ The volumetric ray-march takes classic RK4 steps without error control and
 branches on position (volume exits, opacity exits, step-size rules), so a
 rounding-level change of the path changes the intensity far above rounding:
 the oracle built with the reference's own release flags (-O3 -ffast-math)
-differs from the strict build by a median of ~1e-5.  That self-noise is the
-floor; the all-FP64 GPU path must stay within 20x of it or 2e-5 (and its class
-agreement >= 99.5%).  Mixed path (FP32 radiative transfer with compensated
+differs from the strict build by a median of ~1e-5.  The floor is measured
+directly: the oracle built with one-ulp noise in every component of the Kerr
+right-hand side (oracle variant "ulp") against the strict build.  On the
+rays whose step sequence (outer attempts and steps, volume entries,
+ray-march steps, all in the records) equals the strict oracle's, the GPU
+must stay within 2x (median) and 3x (90th percentile) of that floor, keep
+the oracle's sequence on as many rays as the one-ulp variant does (within
+2%), and agree on the class of >= 99.5% of rays.  Measured: the GPU and the
+one-ulp variant keep the same share of sequences (0.960 / 0.961 RGB,
+0.982 / 0.982 spectral) with the same medians (1.8e-6 / 1.6-2.1e-6 RGB), so
+the GPU's difference from the oracle is rounding noise carried through the
+ray-march, which the reference shows against itself.  Mixed path (FP32 radiative transfer with compensated
 summation): the error against the all-FP64 oracle is measured per frequency
 and held to a median of 1e-3.
 """
@@ -25,7 +34,10 @@ def _setup(grrt, oracle, W, H, spin=0.9, theta=75.0, freqs=None, turbulence=0.4)
     p = grrt.make_params(width=W, height=H, metric=1, spin=spin, observer_theta=math.radians(theta),
                          fov=math.radians(90), volumetric=True, disk_outer=20.0, turbulence=turbulence,
                          seed=42, octaves=2, alpha=0.1, freqs=freqs)
-    T = grrt.volumetric_tables(p)
+    # the tables grrt_create builds (opacity tables on the GPU, which agree
+    # with the host build to ~1e-12 and are deterministic), so the oracle and
+    # the tracer read identical disks
+    T = grrt.volumetric_tables(p, device=True)
     V = oracle.vol_struct(T)
     sc = oracle.scene(width=W, height=H, spin_a=spin, theta_obs=math.radians(theta),
                       fov=math.radians(90), disk=False)
@@ -43,9 +55,29 @@ def _median_rel(a, b):
     return float(np.median(r)) if r.size else 0.0
 
 
+def _same_sequence(g, o):
+    """Rays whose outer attempts and steps, volume entries and ray-march
+    steps all equal the oracle's."""
+    return ((g["attempts"] == o["attempts"]) & (g["steps"] == o["steps"]) & (g["n_cross"] == o["n_cross"]) &
+            (g["pad"] == o["pad"]))
+
+
 def _pct(a, b):
     r = _rel(a, b)
     return " ".join(f"p{q}={np.percentile(r, q):.1e}" for q in (10, 50, 90)) if r.size else "empty"
+
+
+def _noise_floor_check(name, g_vals, o_vals, u_vals, g_seq, u_seq):
+    """GPU against the strict oracle, held to the oracle's own rounding-noise
+    floor (the one-ulp variant against the strict build) on the rays whose
+    step sequence is unchanged: median <= 2x and p90 <= 3x the floor."""
+    e_g = _rel(g_vals[g_seq], o_vals[g_seq])
+    e_u = _rel(u_vals[u_seq], o_vals[u_seq])
+    mg, mu = float(np.median(e_g)), float(np.median(e_u))
+    pg, pu = float(np.percentile(e_g, 90)), float(np.percentile(e_u, 90))
+    print(f"{name}: same sequence GPU median {mg:.1e} p90 {pg:.1e}; one-ulp floor median {mu:.1e} p90 {pu:.1e}",
+          flush=True)
+    assert mg <= 2.0 * max(mu, 1e-9) and pg <= 3.0 * max(pu, 1e-8), (name, mg, mu, pg, pu)
 
 
 def test_volumetric_rgb_parity(grrt, oracle):
@@ -58,15 +90,15 @@ def test_volumetric_rgb_parity(grrt, oracle):
         rec = ctx.records()
     ids = np.arange(64 * 64)
     orec, _ = oracle.vol_trace_samples(sc, V, oracle.JITTER_CENTER, 1, 0, ids)
-    frec, _ = oracle.vol_trace_samples(sc, V, oracle.JITTER_CENTER, 1, 0, ids, variant="fast")
+    urec, _ = oracle.vol_trace_samples(sc, V, oracle.JITTER_CENTER, 1, 0, ids, variant="ulp")
     assert np.mean(rec["cls"] == orec["cls"]) >= 0.995
-    same = (rec["cls"] == orec["cls"]) & (frec["cls"] == orec["cls"])
+    g_seq = _same_sequence(rec, orec) & (rec["cls"] == orec["cls"])
+    u_seq = _same_sequence(urec, orec) & (urec["cls"] == orec["cls"])
+    print(f"rays with the oracle's step sequence: GPU {g_seq.mean():.3f}, one-ulp oracle {u_seq.mean():.3f}",
+          flush=True)
+    assert g_seq.mean() >= u_seq.mean() - 0.02
     for c in range(3):
-        g_vs_o = _median_rel(rec["color"][same, c], orec["color"][same, c])
-        self_noise = _median_rel(frec["color"][same, c], orec["color"][same, c])
-        print(f"channel {c}: GPU vs oracle {_pct(rec['color'][same, c], orec['color'][same, c])}; "
-              f"oracle fast vs strict {_pct(frec['color'][same, c], orec['color'][same, c])}", flush=True)
-        assert g_vs_o <= max(20.0 * self_noise, 2e-5), (c, g_vs_o, self_noise)
+        _noise_floor_check(f"channel {c}", rec["color"][:, c], orec["color"][:, c], urec["color"][:, c], g_seq, u_seq)
     assert np.isfinite(fb).all() and (fb[..., :3] >= 0).all()
     assert np.any(rec["cls"] == grrt.CLASS_DISK)
 
@@ -76,21 +108,24 @@ def test_volumetric_spectral_parity_fp64(grrt, oracle):
     p, V, sc = _setup(grrt, oracle, 48, 48, freqs=FREQS)
     with grrt.Context(p) as ctx:
         ctx.set_frequency_bins(FREQS)
+        ctx.set_records(True)
         cube = ctx.render_spectral()
+        rec = ctx.records()
     ids = np.arange(48 * 48)
-    _, ospec = oracle.vol_trace_samples(sc, V, oracle.JITTER_HASH, 1, 0, ids, freqs=FREQS)
-    _, fspec = oracle.vol_trace_samples(sc, V, oracle.JITTER_HASH, 1, 0, ids, freqs=FREQS, variant="fast")
-    ocube = ospec.reshape(48, 48, 32)
-    fcube = fspec.reshape(48, 48, 32)
+    orec, ospec = oracle.vol_trace_samples(sc, V, oracle.JITTER_HASH, 1, 0, ids, freqs=FREQS)
+    urec, uspec = oracle.vol_trace_samples(sc, V, oracle.JITTER_HASH, 1, 0, ids, freqs=FREQS, variant="ulp")
     assert np.all(cube >= 0) and np.isfinite(cube).all()
+    assert np.mean(rec["cls"] == orec["cls"]) >= 0.995
+    g_seq = _same_sequence(rec, orec) & (rec["cls"] == orec["cls"])
+    u_seq = _same_sequence(urec, orec) & (urec["cls"] == orec["cls"])
+    print(f"rays with the oracle's step sequence: GPU {g_seq.mean():.3f}, one-ulp oracle {u_seq.mean():.3f}",
+          flush=True)
+    assert g_seq.mean() >= u_seq.mean() - 0.02
+    flat = cube.reshape(-1, 32)
     for k in range(32):
-        g_vs_o = _median_rel(cube[..., k], ocube[..., k])
-        self_noise = _median_rel(fcube[..., k], ocube[..., k])
-        if k % 8 == 0:
-            print(f"bin {k}: GPU vs oracle {_pct(cube[..., k], ocube[..., k])}; "
-                  f"self {_pct(fcube[..., k], ocube[..., k])}", flush=True)
-        assert g_vs_o <= max(20.0 * self_noise, 2e-5), (k, g_vs_o, self_noise)
+        _noise_floor_check(f"bin {k}", flat[:, k], ospec[:, k], uspec[:, k], g_seq, u_seq)
     # pixels lit on one side only are the class-boundary grazers
+    ocube = ospec.reshape(48, 48, 32)
     lit = (cube.sum(-1) > 0) != (ocube.sum(-1) > 0)
     assert lit.mean() <= 0.005
 
@@ -139,10 +174,12 @@ def test_spectral_cube_512sq_32bins_full_size(grrt, oracle):
 
 def test_far_field_fp32_error_stated(grrt, oracle):
     """Precision mode 2 (FP32 transfer and FP32 far-field geodesic, the
-    spectral cube's precision policy): the disk entry moves by ~1e-5 M, so
-    each ray's ray-march step sequence changes and per-pixel intensities
-    differ from the all-FP64 oracle by the ray-march's discretisation error;
-    the image-summed spectrum is far less sensitive.  Both are measured and bounded."""
+    spectral cube's precision policy): FP32 outer steps only on outbound
+    far-field segments (r > 15 M, p_r > 0, away from the disk), so the
+    emission is accumulated on FP64 geodesics; the error against the all-FP64
+    oracle is stated per frequency: per-pixel median <= 1e-4, and the
+    image-summed spectrum, which a few bright rays that turn back inwards
+    after an FP32 segment dominate, within 5%."""
     print("case: volumetric spectral 32 bins 48x48, FP32 far field", flush=True)
     p, V, sc = _setup(grrt, oracle, 48, 48, freqs=FREQS)
     with grrt.Context(p) as ctx:
@@ -156,8 +193,8 @@ def test_far_field_fp32_error_stated(grrt, oracle):
     tot = [abs(float(cube[..., k].sum()) - float(ocube[..., k].sum())) / float(ocube[..., k].sum()) for k in range(32)]
     print("per-pixel median rel err:", " ".join(f"{e:.1e}" for e in med), flush=True)
     print("image-summed rel err:", " ".join(f"{e:.1e}" for e in tot), flush=True)
-    assert max(med) <= 0.25
-    assert max(tot) <= 0.1
+    assert max(med) <= 1e-4
+    assert max(tot) <= 0.05
 
 
 def test_spectral_fits_stream_equals_cube(grrt, oracle, tmp_path):
@@ -230,7 +267,7 @@ def test_volumetric_tile_and_spp_groups(grrt, oracle):
 
     W, H = 24, 20
     p = grrt.make_params(width=W, height=H, freqs=FREQS, **kw)
-    V = oracle.vol_struct(grrt.volumetric_tables(p))
+    V = oracle.vol_struct(grrt.volumetric_tables(p, device=True))
     sc = oracle.scene(width=W, height=H, spin_a=0.9, theta_obs=math.radians(75.0), fov=math.radians(90),
                       disk=False)
     with grrt.Context(p) as ctx:
