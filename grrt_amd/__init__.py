@@ -161,6 +161,7 @@ API = [
     ("grrt_ext_get_classes", C.c_int, [_P, _P, C.c_int64]),
     ("grrt_ext_last_stats", C.c_int, [_P, C.POINTER(GRRTStats)]),
     ("grrt_ext_render_device", C.c_int, [_P, _P]),
+    ("grrt_ext_render_device_noalloc", C.c_int, [_P, _P]),
     ("grrt_ext_set_tuning", C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
     ("grrt_ext_render_batch", C.c_int, [_P, C.POINTER(GRRTView), C.c_int, _P, C.c_int]),
     ("grrt_ext_autotune", C.c_int, [_P, C.POINTER(GRRTView), C.c_int, C.POINTER(C.c_int),
@@ -538,6 +539,10 @@ class Context:
 
     def render_device(self, d_ptr: int | None = None) -> int:
         return self._check(self._L.grrt_ext_render_device(self._h, d_ptr), "grrt_ext_render_device")
+
+    def render_device_noalloc(self, d_ptr: int) -> int:
+        """grrt_ext_render_device_noalloc into the device buffer d_ptr."""
+        return self._check(self._L.grrt_ext_render_device_noalloc(self._h, d_ptr), "grrt_ext_render_device_noalloc")
 
     def tonemap_device(self, d_ptr: int | None = None) -> float:
         """Tone-map the context's device frame (or d_ptr) in place; returns

@@ -14,6 +14,7 @@
 #include <thread>
 
 #include "api_internal.hpp"
+#include "noalloc.hpp"
 #include "writers_host.hpp"
 #include "context.hpp"
 #include "grrt/api.h"
@@ -246,11 +247,22 @@ void frame_to_host(GRRTContext* ctx, float* host, const float* dev, size_t n) {
 
 enum class Output { kRGB, kSpectralF64, kSpectralF32 };
 
+// Caller-provided device buffers for an RGB render (the allocation-free
+// entry point): the launch block, counters and per-sample buffers.
+struct FrameScratch {
+    grrt_amd::TraceParams* params;
+    grrt_amd::DevCounters* counters;
+    uint8_t* cls;
+    double* rgb;
+};
+
 // Render the pixel window [x0,x0+tw) x [y0,y0+th) into device memory `out`:
 // RGBA float32 (tw*th*4), or tw*th*nbins spectral values (double or float).
-// Returns 0, or 1 if cancelled.
+// With fs (RGB only, no records) the render uses those buffers and
+// allocates nothing.  Returns 0, or 1 if cancelled.
 int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
-                  grrt_progress_fn progress, void* ud, Output kind = Output::kRGB) {
+                  grrt_progress_fn progress, void* ud, Output kind = Output::kRGB,
+                  const FrameScratch* fs = nullptr) {
     using namespace grrt_amd;
     const auto t_start = std::chrono::steady_clock::now();
     const Resolved& r = ctx->res;
@@ -261,11 +273,15 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
     const long long npix = static_cast<long long>(tw) * th;
     const long long n_rays = npix * nS;
     const int nb = spectral ? static_cast<int>(ctx->frequency_bins.size()) : 3;
-    check(ctx->sample_rgb.ensure(static_cast<size_t>(3 * n_rays)), "hipMalloc(samples)");
-    check(ctx->sample_cls.ensure(static_cast<size_t>(n_rays)), "hipMalloc(classes)");
-    if (ctx->keep_records) check(ctx->records.ensure(static_cast<size_t>(n_rays)), "hipMalloc(records)");
-    check(ctx->counters.ensure(1), "hipMalloc(counters)");
-    check(ctx->params_dev.ensure(1), "hipMalloc(params)");
+    if (fs && (spectral || ctx->keep_records)) throw HipFailure("caller scratch covers RGB renders without records");
+    if (!fs) {
+        check(ctx->sample_rgb.ensure(static_cast<size_t>(3 * n_rays)), "hipMalloc(samples)");
+        check(ctx->sample_cls.ensure(static_cast<size_t>(n_rays)), "hipMalloc(classes)");
+        if (ctx->keep_records) check(ctx->records.ensure(static_cast<size_t>(n_rays)), "hipMalloc(records)");
+        check(ctx->counters.ensure(1), "hipMalloc(counters)");
+        check(ctx->params_dev.ensure(1), "hipMalloc(params)");
+    }
+    TraceParams* const params_dev = fs ? fs->params : ctx->params_dev.ptr;
 
     TraceParams P{};
     fill_view(ctx, P.view, r.a, r.r_obs, r.th_obs, r.ph_obs, r.fov, r.W, r.H, ctx->cam, ctx->flux.ptr,
@@ -279,10 +295,10 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
     P.stars = r.stars;
     P.star_start = ctx->star_start.ptr;
     P.star_tpb = ctx->star_tpb.ptr;
-    P.sample_rgb = ctx->sample_rgb.ptr;
-    P.sample_cls = ctx->sample_cls.ptr;
+    P.sample_rgb = fs ? fs->rgb : ctx->sample_rgb.ptr;
+    P.sample_cls = fs ? fs->cls : ctx->sample_cls.ptr;
     P.records = ctx->keep_records ? ctx->records.ptr : nullptr;
-    P.counters = ctx->counters.ptr;
+    P.counters = fs ? fs->counters : ctx->counters.ptr;
     P.cancel = &ctx->shared_dev->cancel;
     P.claimed = &ctx->shared_dev->claimed;
     P.refill_min = std::clamp(ctx->tune_refill, 1, 64);
@@ -318,14 +334,14 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
     // The stream is idle here (every render ends with a synchronise), so the
     // pinned staging copy can be rewritten.
     *ctx->params_host = P;
-    check(hipMemcpyAsync(ctx->params_dev.ptr, ctx->params_host, sizeof(TraceParams), hipMemcpyHostToDevice,
+    check(hipMemcpyAsync(params_dev, ctx->params_host, sizeof(TraceParams), hipMemcpyHostToDevice,
                          ctx->stream), "upload params");
     check(launch_reset(P.counters, P.sample_cls, n_rays, ctx->stream), "reset kernel");
     check(hipEventRecord(ctx->ev[0], ctx->stream), "hipEventRecord");
-    if (vol) check(launch_trace_vol(ctx->params_dev.ptr, n_rays, nb, spectral, ctx->mixed_precision, ctx->num_cus,
+    if (vol) check(launch_trace_vol(params_dev, n_rays, nb, spectral, ctx->mixed_precision, ctx->num_cus,
                                     ctx->vol_waves_per_simd, ctx->stream),
                    "volumetric trace launch");
-    else check(launch_trace_thin(ctx->params_dev.ptr, 0, blocks, threads, ctx->stream, ctx->integrator),
+    else check(launch_trace_thin(params_dev, 0, blocks, threads, ctx->stream, ctx->integrator),
                "trace kernel launch");
     check(hipEventRecord(ctx->ev[1], ctx->stream), "hipEventRecord");
     if (spectral)
@@ -355,8 +371,10 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
     const hipError_t sync = hipEventSynchronize(ctx->ev[2]);
     ctx->in_flight.store(0);
     check(sync, "render");
-    DevCounters c{};
-    check(hipMemcpy(&c, P.counters, sizeof(c), hipMemcpyDeviceToHost), "read counters");
+    check(hipMemcpyAsync(ctx->counters_host, P.counters, sizeof(DevCounters), hipMemcpyDeviceToHost, ctx->stream),
+          "read counters");
+    check(hipStreamSynchronize(ctx->stream), "read counters");
+    const DevCounters c = *ctx->counters_host;
     float t_trace = 0.0f, t_fin = 0.0f;
     check(hipEventElapsedTime(&t_trace, ctx->ev[0], ctx->ev[1]), "event time");
     check(hipEventElapsedTime(&t_fin, ctx->ev[1], ctx->ev[2]), "event time");
@@ -376,6 +394,8 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
     s.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
     ctx->last_rays = n_rays;
     ctx->last_batch = false;
+    ctx->last_cls = P.sample_cls;
+    ctx->last_rgb = P.sample_rgb;
     ctx->records_valid = ctx->keep_records && !cancelled;
     ctx->progress.store(ctx->progress_hi);
     if (progress) progress(ctx->progress_hi, ud);
@@ -554,6 +574,11 @@ GRRTContext* grrt_create(const GRRTParams* params) {
         void* staging = nullptr;
         check(hipHostMalloc(&staging, sizeof(grrt_amd::TraceParams), hipHostMallocDefault), "hipHostMalloc");
         ctx->params_host = static_cast<grrt_amd::TraceParams*>(staging);
+        void* ch = nullptr;
+        check(hipHostMalloc(&ch, sizeof(grrt_amd::DevCounters), hipHostMallocDefault), "hipHostMalloc");
+        ctx->counters_host = static_cast<grrt_amd::DevCounters*>(ch);
+        // The allocation-free entry point's scratch array on this GPU.
+        check(grrt_amd::noalloc_scratch_init(ctx->device), "hipGetSymbolAddress(scratch)");
         build_scene(ctx);
         return ctx;
     } catch (const std::exception& e) {
@@ -606,6 +631,7 @@ void grrt_destroy(GRRTContext* ctx) {
     ctx->sample_spec.release();
     ctx->spec_out.release();
     if (ctx->params_host) (void)hipHostFree(ctx->params_host);
+    if (ctx->counters_host) (void)hipHostFree(ctx->counters_host);
     if (ctx->shared) (void)hipHostFree(const_cast<grrt_amd::HostShared*>(ctx->shared));
     if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
     for (auto& e : ctx->ev)
@@ -781,7 +807,7 @@ int grrt_ext_get_records(GRRTContext* ctx, GRRTRayRecord* out, int64_t n) {
 int grrt_ext_get_classes(GRRTContext* ctx, uint8_t* out, int64_t n) {
     return guarded(ctx, [&] {
         if (!out || n < 0 || n > ctx->last_rays) throw HipFailure("class count out of range");
-        const uint8_t* src = ctx->last_batch ? ctx->batch_cls.ptr : ctx->sample_cls.ptr;
+        const uint8_t* src = ctx->last_batch ? ctx->batch_cls.ptr : (ctx->last_cls ? ctx->last_cls : ctx->sample_cls.ptr);
         check(hipMemcpy(out, src, static_cast<size_t>(n), hipMemcpyDeviceToHost), "copy classes");
         return 0;
     });
@@ -790,7 +816,7 @@ int grrt_ext_get_classes(GRRTContext* ctx, uint8_t* out, int64_t n) {
 int grrt_ext_get_samples(GRRTContext* ctx, double* rgb, int64_t n) {
     return guarded(ctx, [&] {
         if (!rgb || n < 0 || n > ctx->last_rays) throw HipFailure("sample count out of range");
-        const double* src = ctx->last_batch ? ctx->batch_rgb.ptr : ctx->sample_rgb.ptr;
+        const double* src = ctx->last_batch ? ctx->batch_rgb.ptr : (ctx->last_rgb ? ctx->last_rgb : ctx->sample_rgb.ptr);
         check(hipMemcpy(rgb, src, static_cast<size_t>(n) * 3 * sizeof(double), hipMemcpyDeviceToHost), "copy samples");
         return 0;
     });
@@ -812,6 +838,33 @@ int grrt_ext_render_device(GRRTContext* ctx, float* d_rgba) {
         }
         begin_call(ctx);
         return render_window(ctx, 0, 0, r.W, r.H, dst, nullptr, nullptr);
+    });
+}
+
+int grrt_ext_render_device_noalloc(GRRTContext* ctx, float* d_rgba) {
+    return guarded(ctx, [&] {
+        if (!d_rgba) throw HipFailure("grrt_ext_render_device_noalloc: d_rgba must be a device buffer of width*height*4 floats");
+        if (ctx->keep_records) throw HipFailure("grrt_ext_render_device_noalloc: per-ray records are not kept by this entry point");
+        const auto& r = ctx->res;
+        unsigned char* base = static_cast<unsigned char*>(grrt_amd::noalloc_scratch(ctx->device));
+        if (!base) throw HipFailure("grrt_ext_render_device_noalloc: no scratch on this device");
+        int grid = 1;
+        const long long n = static_cast<long long>(r.W) * r.H * samples_per_pixel(ctx->sampling_mode, ctx->sampling_spp, grid);
+        auto up = [](size_t x) { return (x + 255) & ~size_t{255}; };
+        const size_t o_cnt = up(sizeof(grrt_amd::TraceParams));
+        const size_t o_cls = o_cnt + up(sizeof(grrt_amd::DevCounters));
+        const size_t o_rgb = o_cls + up(static_cast<size_t>(n));
+        const size_t need = o_rgb + static_cast<size_t>(n) * 3 * sizeof(double);
+        if (need > grrt_amd::kNoAllocScratchBytes)
+            throw HipFailure("grrt_ext_render_device_noalloc: " + std::to_string(n) + " samples need " +
+                             std::to_string(need >> 20) + " MiB of scratch; this entry point holds at most " +
+                             std::to_string(grrt_amd::kNoAllocScratchBytes >> 20) +
+                             " MiB (use grrt_ext_render_device)");
+        const FrameScratch fs{reinterpret_cast<grrt_amd::TraceParams*>(base),
+                              reinterpret_cast<grrt_amd::DevCounters*>(base + o_cnt), base + o_cls,
+                              reinterpret_cast<double*>(base + o_rgb)};
+        begin_call(ctx);
+        return render_window(ctx, 0, 0, r.W, r.H, d_rgba, nullptr, nullptr, Output::kRGB, &fs);
     });
 }
 

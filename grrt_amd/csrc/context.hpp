@@ -107,6 +107,7 @@ struct GRRTContext {
     grrt_amd::DevBuf<grrt_amd::DevCounters> counters;
     grrt_amd::DevBuf<grrt_amd::TraceParams> params_dev;   // launch parameters
     grrt_amd::TraceParams* params_host = nullptr;          // pinned staging copy
+    grrt_amd::DevCounters* counters_host = nullptr;        // pinned: counters read back after a launch
     // Host-mapped block: the cancel flag the kernels poll at each queue
     // claim, and the claim position they report back.
     volatile grrt_amd::HostShared* shared = nullptr;
@@ -165,5 +166,9 @@ struct GRRTContext {
 
     GRRTStats last{};
     long long last_rays = 0;
+    // Per-sample buffers of the last single-view render (the context's own,
+    // or the allocation-free entry point's shared scratch).
+    const uint8_t* last_cls = nullptr;
+    const double* last_rgb = nullptr;
     mutable std::atomic<float> progress{1.0f};   // highest progress reported in the current call
 };

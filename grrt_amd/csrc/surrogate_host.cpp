@@ -698,6 +698,8 @@ int grrt_ext_render_surrogate(GRRTContext* ctx, float* out, int device_out, doub
         }
         ctx->last_rays = n_rays;
         ctx->last_batch = false;
+        ctx->last_cls = ctx->sample_cls.ptr;
+        ctx->last_rgb = ctx->sample_rgb.ptr;
         ctx->records_valid = false;
         return 0;
     });

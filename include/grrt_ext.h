@@ -74,6 +74,18 @@ GRRT_EXPORT int grrt_ext_last_stats(const GRRTContext* ctx, GRRTStats* out);
 /* Render the full frame into device memory (d_rgba: width*height*4 floats,
  * or NULL to keep it in the context).  No host copy is made. */
 GRRT_EXPORT int grrt_ext_render_device(GRRTContext* ctx, float* d_rgba);
+/* The same frame as grrt_ext_render_device(ctx, d_rgba) without any device
+ * allocation: no reference counterpart; for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  d_rgba (width*height*4 floats of device memory) is required.
+ * The launch block, counters and per-sample buffers live in a fixed 128 MiB
+ * device array of the library, one per GPU, held from grrt_create on that GPU
+ * until the process ends; frames above its size (25 bytes per sample, about
+ * 5.3 M samples) are rejected with -1 and an error, and per-ray records are
+ * not kept.  Every call on one GPU uses that GPU's array: calls that overlap
+ * on one GPU (two threads, two streams) race.  Volumetric RGB frames are
+ * covered too; spectral renders are not. */
+GRRT_EXPORT int grrt_ext_render_device_noalloc(GRRTContext* ctx, float* d_rgba);
 /* Tone-maps a frame in device memory in place with the grrt_tonemap curve
  * (log-average exposure, extended Reinhard with the frame's white point,
  * gamma 2.2; src/tonemapper.cpp:7-71, whose grrt_tonemap works on a host
