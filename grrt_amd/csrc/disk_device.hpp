@@ -22,7 +22,7 @@ __device__ __forceinline__ double disk_redshift(const ViewParams& V, double rc, 
 
 // Colour added by a crossing at rc seen with redshift g: Novikov–Thorne
 // flux (lerp of the view's table), T = T_peak (F/F_max)^(1/4), observed
-// T g, chromaticity lerp (src/spectrum.cpp:321-330) times (T/1e5)^4 g^4.
+// T g, chromaticity lerp (src/spectrum.cpp:115-124) times (T/1e5)^4 g^4.
 __device__ __forceinline__ void thin_disk_color(const ViewParams& V, const double* flux, const double* s_spec,
                                                 double rc, double g, double col[3]) {
     double F = 0.0;

@@ -41,7 +41,7 @@ CameraFrame build_camera(double M, double a, double r, double th, double ph);
 double build_flux_table(double M, double a, double r_in, double r_out, std::vector<double>& lut);
 
 // Normalised linear-sRGB chromaticity of a blackbody, 1000 temperatures in
-// [1e3, 1e5] K (src/spectrum.cpp:262-302), interleaved RGB.
+// [1e3, 1e5] K (src/spectrum.cpp:56-96), interleaved RGB.
 void build_chromaticity_table(std::vector<double>& lut);
 
 // 5000-star catalogue from mt19937(42) (src/celestial_sphere.cpp:9-46),

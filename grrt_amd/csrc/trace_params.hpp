@@ -9,10 +9,10 @@
 
 namespace grrt_amd {
 
-constexpr int kFluxN = 500;       // src/accretion_disk.h:163 default LUT size
-constexpr int kSpecN = 1000;      // src/spectrum.h:27 default entries
+constexpr int kFluxN = 500;       // include/grrt/scene/accretion_disk.h:14 default LUT size
+constexpr int kSpecN = 1000;      // include/grrt/color/spectrum.h:11 default entries
 constexpr double kSpecTmin = 1000.0, kSpecTmax = 100000.0;
-constexpr int kStarGT = 180, kStarGP = 360;   // src/celestial_sphere.h:123-124
+constexpr int kStarGT = 180, kStarGP = 360;   // include/grrt/scene/celestial_sphere.h:31-32
 constexpr int kStarCells = kStarGT * kStarGP;
 constexpr int kVolBinsPerLane = 4;  // spectral bins held in registers per lane of a ray's group
 constexpr int kVolMaxR = 512;       // radial tables staged in LDS by the volumetric tracer (n_r = 500)
@@ -63,7 +63,7 @@ struct VolDev {
 // the thin disk (its inner edge follows the view's ISCO).
 struct ViewParams {
     double M, a;
-    double r_term;       // horizon radius + 0.01 (src/geodesic_tracer.h:64)
+    double r_term;       // horizon radius + 0.01 (include/grrt/geodesic/geodesic_tracer.h:64)
 
     // Camera: static observer tetrad and the covariant metric there.
     int W, H;

@@ -10,7 +10,7 @@
 //
 // Per-ray semantics follow the reference:
 //   src/geodesic_tracer.cpp:23-181   outer loop, termination, disk crossings
-//   src/rk4.cpp:279-328              adaptive step control
+//   src/rk4.cpp:87-136              adaptive step control
 //   include/grrt/geodesic/rk4.h:73-177  Dormand–Prince stages and error norm
 //   src/camera.cpp:69-96             pixel -> null covector
 //   src/renderer.cpp:14-82           sample placement and averaging
@@ -412,7 +412,7 @@ void trace_thin(const TraceParams* Pg) {
         }
         ++R.attempts;
 
-        // ---- step control (src/rk4.cpp:279-328) ----
+        // ---- step control (src/rk4.cpp:87-136) ----
         // The growth/shrink factor 0.9 (tol/err)^(1/5) is evaluated in FP32
         // (hardware log2/exp2).  It only sets the next trial step length; a
         // relative change d in that length moves the solution by about

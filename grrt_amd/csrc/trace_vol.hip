@@ -719,7 +719,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
     }
 }
 
-// Average the N-bin spectra of each pixel (src/renderer.cpp:235-266); the
+// Average the N-bin spectra of each pixel (src/renderer.cpp:84-137); the
 // output is double (ABI) or float (cube) per `out_f32`.
 __global__ void finalize_spectral(const double* __restrict__ spec, const uint8_t* __restrict__ cls, void* out,
                                   long long npix, int nS, int nb, double inv, int out_f32) {

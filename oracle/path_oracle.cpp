@@ -271,7 +271,7 @@ void rk4_step(double M, double a, const State& y, double dl, State& out) {
     }
 }
 
-// src/rk4.cpp:279-328 — PI-controlled adaptive DP45; returns next dl.
+// src/rk4.cpp:87-136 — PI-controlled adaptive DP45; returns next dl.
 double adaptive_step(double M, double a, const State& y, double dlambda, double tol,
                      State& out, int& attempts) {
     const double dl_min = 1e-6, safety = 0.9, p_exp = 0.2, grow_max = 5.0, shrink_max = 0.2;
@@ -485,14 +485,14 @@ const double kCIE[81][3] = {
 constexpr int kSpecN = 1000;
 constexpr double kTmin = 1000.0, kTmax = 100000.0;
 
-double planck_lambda(double lambda, double T) {  // src/spectrum.cpp:255-260 (SI)
+double planck_lambda(double lambda, double T) {  // src/spectrum.cpp:49-54 (SI)
     const double h = 6.62607015e-34, c = 2.99792458e8, k = 1.380649e-23;
     const double x = h * c / (lambda * k * T);
     if (x > 500.0) return 0.0;
     return (2.0 * h * c * c) / (lambda * lambda * lambda * lambda * lambda * (std::exp(x) - 1.0));
 }
 
-void build_spectrum(std::vector<double>& lut) {  // src/spectrum.cpp:262-302
+void build_spectrum(std::vector<double>& lut) {  // src/spectrum.cpp:56-96
     lut.assign(3 * kSpecN, 0.0);
     for (int i = 0; i < kSpecN; ++i) {
         const double t = kTmin + (kTmax - kTmin) * i / (kSpecN - 1);
@@ -640,7 +640,7 @@ const Stars& star_catalog() {
     return s;
 }
 
-void chroma(double T, double out[3]) {  // src/spectrum.cpp:321-330
+void chroma(double T, double out[3]) {  // src/spectrum.cpp:115-124
     const auto& lut = spectrum_lut();
     T = sclamp(T, kTmin, kTmax);
     const double frac = (T - kTmin) / (kTmax - kTmin) * (kSpecN - 1);
