@@ -341,6 +341,7 @@ def spectral_cube_section(grrt, reps: int, cpu: bool) -> dict:
                              "opacity_device": round(builds["device"][0], 2),
                              "total_host": round(builds["host"][1], 2),
                              "total_device": round(builds["device"][1], 2),
+                             # the library's own table build uses min(16, hardware threads)
                              "host_threads": max(1, min(16, os.cpu_count() or 1))}
     for mode, name in ((1, "mixed"), (0, "fp64"), (2, "mixed_far_field_fp32")):
         ctx.set_precision(mode)
@@ -379,22 +380,21 @@ def spectral_cube_section(grrt, reps: int, cpu: bool) -> dict:
         V = O.vol_struct(T)
         sc = O.scene(width=512, height=512, spin_a=0.9, theta_obs=math.radians(75.0), fov=math.radians(90.0),
                      disk=False)
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads, host = host_cores()
         rng = np.random.default_rng(2025)
         probe = np.sort(rng.choice(rays, 1000, replace=False))
         t1 = time.perf_counter()
         O.vol_trace_samples(sc, V, O.JITTER_HASH, 1, 0, probe, freqs=CUBE_FREQS, threads=threads, variant="fast")
         rate = len(probe) / (time.perf_counter() - t1)
-        n = int(min(rays, max(1000, rate * 10.0)))
+        n = int(min(rays, max(1000, rate * 4.0)))
         ids = np.sort(rng.choice(rays, n, replace=False))
-        t1 = time.perf_counter()
-        O.vol_trace_samples(sc, V, O.JITTER_HASH, 1, 0, ids, freqs=CUBE_FREQS, threads=threads, variant="fast")
-        dt = time.perf_counter() - t1
+        dt = timed_median(lambda: O.vol_trace_samples(sc, V, O.JITTER_HASH, 1, 0, ids, freqs=CUBE_FREQS,
+                                                      threads=threads, variant="fast"))
         cube_s = rays / (n / dt)
         out["cpu_baseline"] = {"value": round(n / dt / 1e6, 5), "unit": "Mrays/s", "cores": threads,
-                               "kind": "port", "cube_s": round(cube_s, 2),
-                               "sample": f"{n} rays drawn uniformly from the cube's 512^2, {dt:.1f} s, oracle "
-                                         f"built -O3 -ffast-math, all FP64, OpenMP guided"}
+                               "kind": "port", "cube_s": round(cube_s, 2), "host": host,
+                               "sample": f"{n} rays drawn uniformly from the cube's 512^2, median of 3 runs of "
+                                         f"{dt:.1f} s, oracle built -O3 -ffast-math, all FP64, OpenMP guided"}
         out["speedup_mixed_vs_cpu"] = round(cube_s * 1e3 / out["mixed"]["ms_per_cube"], 1)
     return out
 
@@ -467,7 +467,8 @@ def main() -> int:
     tm_gbs = W * H * 44 / (tm_ms / 1e3) / 1e9
 
     out = {
-        "metric": "Mrays/s Kerr a=0.998 1024^2 4spp (plain FP64 form)",
+        "metric": "Mrays/s Kerr a=0.998 1024\u00b2 4spp: plain FP64 vs surrogate @\u22641% flux err; %VALU pk",
+        # value: the plain FP64 form; the surrogate form is under "surrogate"
         "value": round(value, 3),
         "unit": "Mrays/s",
         "n_gpus": 1,

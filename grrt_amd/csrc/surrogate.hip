@@ -325,12 +325,42 @@ __device__ __forceinline__ void interleave_window() {
 // row from the image centre outwards (H/2, H/2-1, H/2+1, ...) so the rays
 // around the shadow, whose fallback traces are the longest, are queued for
 // the integrator first.
-__device__ __forceinline__ long long visit_ray(const InferArgs& IA, long long l) {
-    if (IA.row_rays <= 0) return IA.ray_base + l;
-    const long long k = l / IA.row_rays, col = l - k * IA.row_rays;
+//
+// The visit positions of a wavefront's lanes are p0 + off with p0 the same
+// for the whole wavefront and off < 64: one 64-bit division (on the scalar
+// unit) per wavefront, then each lane steps across row ends by subtraction.
+__device__ __forceinline__ long long visit_ray(const InferArgs& IA, long long p0, int off) {
+    const long long p0u = static_cast<long long>(
+        (static_cast<unsigned long long>(__builtin_amdgcn_readfirstlane(static_cast<int>(p0 >> 32))) << 32) |
+        static_cast<unsigned>(__builtin_amdgcn_readfirstlane(static_cast<int>(p0))));
+    if (IA.row_rays <= 0) return IA.ray_base + p0u + off;
+    const long long rr = IA.row_rays;
+    long long k = p0u / rr;
+    long long col = p0u - k * rr + off;
+    while (col >= rr) {
+        col -= rr;
+        ++k;
+    }
     const long long half = IA.rows / 2;
     const long long row = (k & 1) ? half - (k + 1) / 2 : half + k / 2;
-    return IA.ray_base + row * IA.row_rays + col;
+    return IA.ray_base + row * rr + col;
+}
+
+// decode_ray for a whole-view window (no tile order) in 32-bit arithmetic
+// while the ray index fits; the general 64-bit form otherwise.
+__device__ __forceinline__ void decode_view_ray(const ViewParams& V, long long q, SampleId& d) {
+    if (V.tile <= 0 && q >= 0 && q < 0x7fffffffLL) {
+        const unsigned uq = static_cast<unsigned>(q), nS = static_cast<unsigned>(V.nS);
+        const unsigned tw = static_cast<unsigned>(V.tw);
+        const unsigned pl = uq / nS;
+        const unsigned pj = pl / tw;
+        d.s = static_cast<int>(uq - pl * nS);
+        d.i = V.x0 + static_cast<int>(pl - pj * tw);
+        d.j = V.y0 + static_cast<int>(pj);
+        d.local = q;
+    } else {
+        decode_ray(V, q, d);
+    }
 }
 
 // One 32-neuron block of a layer from a weight window: the bias, then the
@@ -360,7 +390,14 @@ __device__ __forceinline__ void silu_block(const f32x16& a, bf16x8& lo_slot, bf1
 }
 
 __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
-    __shared__ uint4 wbuf[2][kChunkVec];
+    // The two weight-window buffers are separate LDS variables: the compiler
+    // gives each its own alias scope, so reads of the current window are not
+    // made to wait for the direct-to-LDS loads filling the other one (with a
+    // single [2][n] array indexed by a run-time parity it waits, and every
+    // window paid the full L2 load latency).  Window c (0..23 in a pass, an
+    // even count) reads buffer c % 2 and fills the other.
+    __shared__ uint4 wbuf0[kChunkVec];
+    __shared__ uint4 wbuf1[kChunkVec];
     __shared__ uint4 s_pack0[kPack0 / 8];
     __shared__ uint4 s_pack4[kPack4 / 8];
     __shared__ float s_bias[4 * kHid + 16];
@@ -384,10 +421,9 @@ __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
     if (tid == 0) s_claim = static_cast<long long>(atomicAdd(IA.tile_counter, 1ull));
     __syncthreads();
     long long tile = s_claim;
-    if (tile < n_tiles) load_window(chunk_src(A, 0), wbuf[0]);
+    if (tile < n_tiles) load_window(chunk_src(A, 0), wbuf0);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    int buf = 0;
     while (tile < n_tiles) {
         if (tid == 0) s_claim = static_cast<long long>(atomicAdd(IA.tile_counter, 1ull));
         bool more_tiles = true;         // read from s_claim after the first window's barrier
@@ -397,21 +433,24 @@ __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
             // View constants are re-read per tile (cached) rather than held
             // in registers across the network.
             asm volatile("" ::: "memory");
-            long long i = ray0 + lane;
-            if (i >= n) i = n - 1;
+            // Lanes past the end repeat the last ray.
+            const long long p0 = ray0 < n ? ray0 : n - 1;
+            const int off = static_cast<int>((ray0 + lane < n ? ray0 + lane : n - 1) - p0);
+            const long long vi = visit_ray(IA, p0, off);
             float xl[kIn];
             if (IA.x_in) {
-                const long long vi = visit_ray(IA, i);
 #pragma unroll
                 for (int k = 0; k < kIn; ++k) xl[k] = IA.x_in[vi * kIn + k];
             } else {
                 SampleId d;
-                decode_ray(V, visit_ray(IA, i), d);
+                decode_view_ray(V, vi, d);
                 double px, py;
                 sample_position(V, d, px, py);
-                const double alpha = (px - V.W / 2.0) * V.fov / V.W;
-                const double beta = (py - V.H / 2.0) * V.fov / V.W;
-                features(alpha, beta, V.a / V.M, cos(V.th_obs), V.r_obs / V.M, xl);
+                xl[0] = static_cast<float>((px - 0.5 * V.W) * IA.feat_scale);
+                xl[1] = static_cast<float>((py - 0.5 * V.H) * IA.feat_scale);
+                xl[2] = IA.feat_view[0];
+                xl[3] = IA.feat_view[1];
+                xl[4] = IA.feat_view[2];
             }
 #pragma unroll
             for (int k = 0; k < kIn; ++k) s_x[wave][lane][k] = xl[k];
@@ -452,11 +491,13 @@ __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
         auto window = [&](int cidx, auto&& body) {
             if (cidx == kChunks - 1 && t == 1) more_tiles = s_claim < n_tiles;
             const bool has_next = cidx + 1 < kChunks || t == 0 || more_tiles;
-            if (has_next) load_window(chunk_src(A, cidx + 1 < kChunks ? cidx + 1 : 0), wbuf[buf ^ 1]);
-            body(reinterpret_cast<const bf16x8*>(wbuf[buf]));
+            static_assert(kChunks % 2 == 0, "window parity must repeat from pass to pass");
+            uint4* const cur = (cidx & 1) ? wbuf1 : wbuf0;
+            uint4* const nxt = (cidx & 1) ? wbuf0 : wbuf1;
+            if (has_next) load_window(chunk_src(A, cidx + 1 < kChunks ? cidx + 1 : 0), nxt);
+            body(reinterpret_cast<const bf16x8*>(cur));
             __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
-            buf ^= 1;
         };
         auto hidden = [&](int layer, const bf16x8 (&in)[16], bf16x8 (&out)[16]) {
             f32x16 prev;
@@ -517,7 +558,7 @@ __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
         asm volatile("" ::: "memory");
         // One lane per ray: ray ray0 + lane.
         const bool valid = ray0 + lane < n;
-        const long long i = visit_ray(IA, valid ? ray0 + lane : 0);
+        const long long i = visit_ray(IA, ray0 < n ? ray0 : 0, valid ? lane : 0);
         float y[kOut];
 #pragma unroll
         for (int m = 0; m < kOut; ++m) y[m] = outs[wave][lane][m];
@@ -546,7 +587,7 @@ __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
         bool inconsistent = false;
         if (valid && cls == GRRT_CLASS_DISK) {
             SampleId d;
-            decode_ray(V, i, d);
+            decode_view_ray(V, i, d);
             double P0, P3;
             Vec5 y0;
             camera_ray(V, d, P0, P3, y0);

@@ -119,6 +119,11 @@ struct InferArgs {
     // y_raw[i][9] and neither shades nor lists any ray.
     const float* x_in;
     float* y_raw;
+    // Per-view inputs, formed once on the host with features(): the pixel
+    // scale fov / (W * kHalfFov) that turns (px - W/2) into input 0 (and
+    // (py - H/2) into input 1), and inputs 2..4 (spin, cos theta, ln r).
+    double feat_scale;
+    float feat_view[3];
 };
 
 hipError_t launch_pack(const float* params, __bf16* pack0, __bf16* packH, __bf16* pack4, __bf16* WT,

@@ -182,6 +182,18 @@ TraceParams frame_params(GRRTContext* ctx, long long n_rays) {
     return P;
 }
 
+// The pass's per-view inputs (features() of the view's spin, inclination
+// and distance) and the pixel scale of inputs 0 and 1, formed once here
+// instead of per ray.
+void set_view_features(InferArgs& IA, const ViewParams& V) {
+    float x[kIn];
+    features(0.0, 0.0, V.a / V.M, std::cos(V.th_obs), V.r_obs / V.M, x);
+    IA.feat_view[0] = x[2];
+    IA.feat_view[1] = x[3];
+    IA.feat_view[2] = x[4];
+    IA.feat_scale = V.fov / (static_cast<double>(V.W) * FeatureBox::kHalfFov);
+}
+
 }  // namespace
 
 extern "C" {
@@ -218,6 +230,7 @@ int grrt_ext_surrogate_infer_raw(GRRTContext* ctx, const float* x, int64_t n, fl
         IA.tile_counter = S.counts.ptr + 3;
         IA.x_in = x ? dx.ptr : nullptr;
         IA.y_raw = dy.ptr;
+        set_view_features(IA, P.view);
         check(launch_mlp_infer(IA, ctx->num_cus, ctx->stream), "surrogate pass");
         check(hipStreamSynchronize(ctx->stream), "surrogate pass");
         check(hipMemcpy(y, dy.ptr, sizeof(float) * kOut * n, hipMemcpyDeviceToHost), "copy y");
@@ -605,12 +618,12 @@ int grrt_ext_render_surrogate(GRRTContext* ctx, float* out, int device_out, doub
         for (int k = 1; k < nb; ++k) check(launch_reset(ctx->counters.ptr + k, nullptr, 0, st), "reset");
         check(launch_reset_counts(S.counts.ptr, 4 * nb, st), "reset counts");
         check(hipEventRecord(ctx->ev[0], st), "event");
-        // The fallback list is short and led by its longest integrations; one
-        // block per CU (one wavefront per SIMD) lets those run without a
-        // second wavefront taking half of their SIMD's issue slots (the
-        // 1024^2 x 4 frame: 3.39 ms after the network pass, 3.78 ms with
-        // the plain frame's two blocks per CU).
-        const int blocks = std::max(1, ctx->num_cus);
+        // The fallback launch takes the plain frame's launch shape (three
+        // blocks per CU by default).  On the 2048^2 x 4 frame (~0.77 M listed
+        // rays) that traces them in 5.7 ms against 6.5 ms with one block per
+        // CU; on the 1024^2 x 4 frame (~0.19 M) the two are within run-to-run
+        // spread (3.2-3.5 ms).
+        const int blocks = std::max(1, ctx->num_cus * ctx->tune_blocks_per_cu);
         for (int k = 0; k < nb; ++k) {
             InferArgs IA{};
             IA.mlp = net_args(S);
@@ -628,6 +641,7 @@ int grrt_ext_render_surrogate(GRRTContext* ctx, float* out, int device_out, doub
             IA.counts = S.counts.ptr + 4 * k;
             IA.judge_cap = bands[k].cap;
             IA.tile_counter = S.counts.ptr + 4 * k + 3;
+            set_view_features(IA, P.view);
             check(launch_mlp_infer(IA, ctx->num_cus, st), "surrogate pass");
             check(hipEventRecord(ctx->band_ev[k], st), "event");
             check(hipStreamWaitEvent(s2, ctx->band_ev[k], 0), "stream wait");

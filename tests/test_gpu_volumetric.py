@@ -272,17 +272,27 @@ def test_volumetric_tile_and_spp_groups(grrt, oracle):
                       disk=False)
     with grrt.Context(p) as ctx:
         ctx.set_frequency_bins(FREQS)
+        ctx.set_records(True)
         cube = ctx.render_spectral()
+        rec = ctx.records()
     ids = np.arange(W * H * 4)
-    _, ospec = oracle.vol_trace_samples(sc, V, oracle.JITTER_HASH, 4, 0, ids, freqs=FREQS)
-    _, fspec = oracle.vol_trace_samples(sc, V, oracle.JITTER_HASH, 4, 0, ids, freqs=FREQS, variant="fast")
+    orec, ospec = oracle.vol_trace_samples(sc, V, oracle.JITTER_HASH, 4, 0, ids, freqs=FREQS)
+    urec, uspec = oracle.vol_trace_samples(sc, V, oracle.JITTER_HASH, 4, 0, ids, freqs=FREQS, variant="ulp")
     ocube = ospec.reshape(H, W, 4, 32).mean(2)
-    fcube = fspec.reshape(H, W, 4, 32).mean(2)
+    ucube = uspec.reshape(H, W, 4, 32).mean(2)
     assert np.any(ocube > 0)
-    for k in range(0, 32, 4):
-        g_vs_o = _median_rel(cube[..., k], ocube[..., k])
-        self_noise = _median_rel(fcube[..., k], ocube[..., k])
-        assert g_vs_o <= max(20.0 * self_noise, 2e-5), (k, g_vs_o, self_noise)
+    assert np.mean(rec["cls"] == orec["cls"]) >= 0.995
+    # a pixel keeps the oracle's sequence when all four of its samples do
+    g_seq = (_same_sequence(rec, orec) & (rec["cls"] == orec["cls"])).reshape(-1, 4).all(1)
+    u_seq = (_same_sequence(urec, orec) & (urec["cls"] == orec["cls"])).reshape(-1, 4).all(1)
+    print(f"pixels with the oracle's step sequences: GPU {g_seq.mean():.3f}, one-ulp oracle {u_seq.mean():.3f}",
+          flush=True)
+    assert g_seq.mean() >= u_seq.mean() - 0.02
+    # 480 pixels give too few lit ones per bin for a stable median, so the
+    # bins are pooled (pixel x bin) into one noise-floor check.
+    g_all, o_all, u_all = (c.reshape(-1, 32) for c in (cube, ocube, ucube))
+    _noise_floor_check("4-spp, all 32 bins pooled", g_all.ravel(), o_all.ravel(), u_all.ravel(),
+                       np.repeat(g_seq, 32), np.repeat(u_seq, 32))
 
 
 def test_volumetric_step_doubling_parity(grrt, oracle):
@@ -299,13 +309,16 @@ def test_volumetric_step_doubling_parity(grrt, oracle):
         rec = ctx.records()
     ids = np.arange(48 * 48)
     orec, _ = oracle.vol_trace_samples(sc, V, oracle.JITTER_CENTER, 1, 0, ids)
-    frec, _ = oracle.vol_trace_samples(sc, V, oracle.JITTER_CENTER, 1, 0, ids, variant="fast")
+    urec, _ = oracle.vol_trace_samples(sc, V, oracle.JITTER_CENTER, 1, 0, ids, variant="ulp")
     assert np.mean(rec["cls"] == orec["cls"]) >= 0.995
-    same = (rec["cls"] == orec["cls"]) & (frec["cls"] == orec["cls"])
+    g_seq = _same_sequence(rec, orec) & (rec["cls"] == orec["cls"])
+    u_seq = _same_sequence(urec, orec) & (urec["cls"] == orec["cls"])
+    print(f"rays with the oracle's step sequence: GPU {g_seq.mean():.3f}, one-ulp oracle {u_seq.mean():.3f}",
+          flush=True)
+    assert g_seq.mean() >= u_seq.mean() - 0.02
     for c in range(3):
-        g_vs_o = _median_rel(rec["color"][same, c], orec["color"][same, c])
-        self_noise = _median_rel(frec["color"][same, c], orec["color"][same, c])
-        assert g_vs_o <= max(20.0 * self_noise, 2e-5), (c, g_vs_o, self_noise)
+        _noise_floor_check(f"step-doubling channel {c}", rec["color"][:, c], orec["color"][:, c],
+                           urec["color"][:, c], g_seq, u_seq)
     assert np.any(rec["cls"] == grrt.CLASS_DISK)
 
 
