@@ -171,6 +171,7 @@ API = [
     ("grrt_ext_write_fits_cube", C.c_int, [C.c_char_p, _P, C.c_int, C.c_int, C.c_int, C.c_int,
                                            C.POINTER(C.c_double), C.POINTER(GRRTParams)]),
     ("grrt_ext_write_hdr", C.c_int, [C.c_char_p, C.POINTER(C.c_float), C.c_int, C.c_int, C.c_int]),
+    ("grrt_ext_render_hdr", C.c_int, [_P, C.c_char_p, C.c_char_p, C.POINTER(C.c_float), PROGRESS_FN, C.c_void_p]),
     ("grrt_ext_surrogate_init", C.c_int, [_P, C.c_uint64]),
     ("grrt_ext_surrogate_generate", C.c_int, [_P, C.c_int64, C.c_uint64, C.c_int]),
     ("grrt_ext_surrogate_train", C.c_int, [_P, C.POINTER(GRRTTrainConfig), C.POINTER(GRRTTrainStats)]),
@@ -390,6 +391,18 @@ class Context:
         fn = self._L.grrt_ext_render_spectral_to_fits_f32 if f32 else self._L.grrt_render_spectral_to_fits_cb
         return self._check(fn(self._h, path.encode(), self.params.width, self.params.height, cb, None),
                            "render_spectral_to_fits")
+
+    def render_hdr(self, path_linear: str, path_scaled: str | None = None, progress=None):
+        """grrt_ext_render_hdr: render with the rows streamed to the linear
+        Radiance file (and the exposure-scaled one after it); returns
+        (rc, frame) with rc 0, or 1 if cancelled."""
+        fb = np.zeros((self.params.height, self.params.width, 4), np.float32)
+        cb = PROGRESS_FN(lambda f, _ud: progress(f)) if progress else PROGRESS_FN(0)
+        rc = self._check(self._L.grrt_ext_render_hdr(self._h, path_linear.encode(),
+                                                      path_scaled.encode() if path_scaled else None,
+                                                      fb.ctypes.data_as(C.POINTER(C.c_float)), cb, None),
+                         "render_hdr")
+        return rc, fb
 
     # ---- learned surrogate ----
     def surrogate_init(self, seed: int = 0):

@@ -539,6 +539,76 @@ int spectral_to_fits(GRRTContext* ctx, const char* path, int width, int height, 
     return rc;
 }
 
+// The CLI's two HDR outputs (cli/main.cpp:398-434) with the linear one
+// streamed: the frame is rendered in bands of rows, and while the GPU renders
+// band k+1 a writer thread encodes band k and appends its rows to the linear
+// file and to the exposure statistics (in pixel order, so the scale equals a
+// pass over the whole frame).  The exposure-scaled file is written from the
+// finished frame.  A cancel request ends the stream after the current band:
+// the linear file is completed with black rows, the rows not rendered are
+// zero in the framebuffer, no scaled file is written, and the call returns 1.
+int render_hdr(GRRTContext* ctx, const char* path_linear, const char* path_scaled, float* framebuffer,
+               grrt_progress_fn progress, void* ud) {
+    if (!path_linear || path_linear[0] == '\0') throw HipFailure("grrt_ext_render_hdr: linear output path is empty");
+    const auto& r = ctx->res;
+    const int W = r.W, H = r.H;
+    begin_call(ctx);
+    std::vector<float> own;
+    float* fb = framebuffer;
+    if (!fb) {
+        own.assign(static_cast<size_t>(W) * H * 4, 0.0f);
+        fb = own.data();
+    }
+    // About 2^18 pixels per band.
+    const int band = std::clamp((1 << 18) / std::max(W, 1), 1, H);
+    check(ctx->frame.ensure(4 * static_cast<size_t>(W) * band), "hipMalloc(frame band)");
+    grrt_amd::RgbeWriter lin(path_linear, W, H);
+    grrt_amd::ExposureSum exposure;
+    std::thread writer;
+    std::exception_ptr writer_error;
+    // The writer thread is joined on every way out of this function.
+    struct Join {
+        std::thread& t;
+        ~Join() {
+            if (t.joinable()) t.join();
+        }
+    } join{writer};
+    int rc = 0;
+    int y = 0;
+    for (; y < H; y += band) {
+        if (*ctx->cancel_flag) { rc = 1; break; }
+        const int bh = std::min(band, H - y);
+        ctx->progress_lo = static_cast<float>(y) / static_cast<float>(H);
+        ctx->progress_hi = static_cast<float>(y + bh) / static_cast<float>(H);
+        rc = render_window(ctx, 0, y, W, bh, ctx->frame.ptr, nullptr, nullptr);
+        if (rc != 0) break;
+        if (writer.joinable()) writer.join();
+        if (writer_error) std::rethrow_exception(writer_error);
+        float* rows = fb + static_cast<size_t>(y) * W * 4;
+        frame_to_host(ctx, rows, ctx->frame.ptr, 4 * static_cast<size_t>(W) * bh);
+        writer = std::thread([&lin, &exposure, &writer_error, rows, W, bh] {
+            try {
+                exposure.add(rows, static_cast<long long>(W) * bh);
+                lin.write_rows(rows, bh, 1.0f);
+            } catch (...) {
+                writer_error = std::current_exception();
+            }
+        });
+        const float f = static_cast<float>(y + bh) / static_cast<float>(H);
+        ctx->progress.store(f);
+        if (progress) progress(f, ud);
+    }
+    if (writer.joinable()) writer.join();
+    if (writer_error) std::rethrow_exception(writer_error);
+    if (rc != 0 && y < H) std::fill(fb + static_cast<size_t>(y) * W * 4, fb + static_cast<size_t>(H) * W * 4, 0.0f);
+    lin.finish();
+    if (rc == 0 && path_scaled && path_scaled[0] != '\0') grrt_amd::write_rgbe(path_scaled, fb, W, H, exposure.scale());
+    ctx->progress_lo = 0.0f;
+    ctx->progress_hi = 1.0f;
+    if (rc == 0) ctx->progress.store(1.0f);
+    return rc;
+}
+
 }  // namespace grrt_api
 
 
@@ -1335,6 +1405,11 @@ int grrt_ext_write_fits_cube(const char* path, const void* cube, int bitpix, int
         g_last_error = e.what();
         return -1;
     }
+}
+
+int grrt_ext_render_hdr(GRRTContext* ctx, const char* path_linear, const char* path_scaled, float* framebuffer,
+                        grrt_progress_fn progress, void* user_data) {
+    return guarded(ctx, [&] { return render_hdr(ctx, path_linear, path_scaled, framebuffer, progress, user_data); });
 }
 
 int grrt_ext_write_hdr(const char* path, const float* rgba, int width, int height, int normalised) {

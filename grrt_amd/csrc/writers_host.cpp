@@ -204,15 +204,34 @@ void FitsCubeWriter::finish() {
     fd_ = -1;
 }
 
-void write_rgbe(const std::string& path, const float* rgba, int width, int height, float scale) {
-    if (!rgba || width <= 0 || height <= 0) throw std::runtime_error("empty image");
-    std::FILE* f = std::fopen(path.c_str(), "wb");
-    if (!f) throw std::runtime_error("cannot open '" + path + "' for writing: " + std::strerror(errno));
-    std::fprintf(f, "#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y %d +X %d\n", height, width);
-    std::vector<unsigned char> px(static_cast<size_t>(width) * 4), out;
-    out.reserve(static_cast<size_t>(width) * 5 + 4);
-    for (int j = 0; j < height; ++j) {
+RgbeWriter::RgbeWriter(const std::string& path, int width, int height) : width_(width), height_(height), path_(path) {
+    if (width <= 0 || height <= 0) throw std::runtime_error("empty image");
+    f_ = std::fopen(path.c_str(), "wb");
+    if (!f_) throw std::runtime_error("cannot open '" + path + "' for writing: " + std::strerror(errno));
+    if (std::fprintf(f_, "#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y %d +X %d\n", height, width) < 0) {
+        std::fclose(f_);
+        f_ = nullptr;
+        throw std::runtime_error("write error on '" + path + "'");
+    }
+    px_.resize(static_cast<size_t>(width) * 4);
+    out_.reserve(static_cast<size_t>(width) * 5 + 4);
+}
+
+RgbeWriter::~RgbeWriter() {
+    if (f_) std::fclose(f_);
+}
+
+void RgbeWriter::write_rows(const float* rgba, int rows, float scale) {
+    if (!f_) throw std::runtime_error("RGBE writer is closed");
+    if (rows < 0 || next_ + rows > height_) throw std::runtime_error("RGBE rows past the image height");
+    const int width = width_;
+    for (int j = 0; j < rows; ++j, ++next_) {
         for (int i = 0; i < width; ++i) {
+            unsigned char* q = &px_[static_cast<size_t>(i) * 4];
+            if (!rgba) {
+                q[0] = q[1] = q[2] = q[3] = 0;
+                continue;
+            }
             const float* p = rgba + (static_cast<size_t>(j) * width + i) * 4;
             float c[3];
             for (int k = 0; k < 3; ++k) {
@@ -220,7 +239,6 @@ void write_rgbe(const std::string& path, const float* rgba, int width, int heigh
                 c[k] = (v > 0.0f && std::isfinite(v)) ? v : 0.0f;   // RGBE holds non-negative finite values
             }
             const float m = std::max(c[0], std::max(c[1], c[2]));
-            unsigned char* q = &px[static_cast<size_t>(i) * 4];
             if (m < 1e-32f) {
                 q[0] = q[1] = q[2] = q[3] = 0;
             } else {
@@ -230,35 +248,44 @@ void write_rgbe(const std::string& path, const float* rgba, int width, int heigh
                 q[3] = static_cast<unsigned char>(e + 128);
             }
         }
-        out.clear();
+        out_.clear();
         if (width >= 8 && width < 32768) {
             // New-style scanline: marker, then each channel as literal runs of <= 128 bytes.
-            out.push_back(2);
-            out.push_back(2);
-            out.push_back(static_cast<unsigned char>(width >> 8));
-            out.push_back(static_cast<unsigned char>(width & 255));
+            out_.push_back(2);
+            out_.push_back(2);
+            out_.push_back(static_cast<unsigned char>(width >> 8));
+            out_.push_back(static_cast<unsigned char>(width & 255));
             for (int ch = 0; ch < 4; ++ch)
                 for (int i = 0; i < width;) {
                     const int n = std::min(128, width - i);
-                    out.push_back(static_cast<unsigned char>(n));
-                    for (int t = 0; t < n; ++t) out.push_back(px[static_cast<size_t>(i + t) * 4 + ch]);
+                    out_.push_back(static_cast<unsigned char>(n));
+                    for (int t = 0; t < n; ++t) out_.push_back(px_[static_cast<size_t>(i + t) * 4 + ch]);
                     i += n;
                 }
         } else {
-            out.assign(px.begin(), px.end());
+            out_.assign(px_.begin(), px_.end());
         }
-        if (std::fwrite(out.data(), 1, out.size(), f) != out.size()) {
-            std::fclose(f);
-            throw std::runtime_error("write error on '" + path + "'");
-        }
+        if (std::fwrite(out_.data(), 1, out_.size(), f_) != out_.size())
+            throw std::runtime_error("write error on '" + path_ + "'");
     }
-    if (std::fclose(f) != 0) throw std::runtime_error("close error on '" + path + "'");
 }
 
-float hdr_exposure_scale(const float* rgba, int width, int height) {
-    double log_sum = 0.0;
-    long long lit = 0;
-    const long long n = static_cast<long long>(width) * height;
+void RgbeWriter::finish() {
+    if (!f_) return;
+    if (next_ < height_) write_rows(nullptr, height_ - next_, 1.0f);
+    const int rc = std::fclose(f_);
+    f_ = nullptr;
+    if (rc != 0) throw std::runtime_error("close error on '" + path_ + "'");
+}
+
+void write_rgbe(const std::string& path, const float* rgba, int width, int height, float scale) {
+    if (!rgba || width <= 0 || height <= 0) throw std::runtime_error("empty image");
+    RgbeWriter w(path, width, height);
+    w.write_rows(rgba, height, scale);
+    w.finish();
+}
+
+void ExposureSum::add(const float* rgba, long long n) {
     for (long long i = 0; i < n; ++i) {
         const double L = 0.2126 * rgba[4 * i] + 0.7152 * rgba[4 * i + 1] + 0.0722 * rgba[4 * i + 2];
         if (L > 1e-10) {
@@ -266,8 +293,17 @@ float hdr_exposure_scale(const float* rgba, int width, int height) {
             ++lit;
         }
     }
+}
+
+float ExposureSum::scale() const {
     const double L_avg = lit > 0 ? std::exp(log_sum / lit) : 1.0;
     return static_cast<float>(0.18 / L_avg);
+}
+
+float hdr_exposure_scale(const float* rgba, int width, int height) {
+    ExposureSum s;
+    s.add(rgba, static_cast<long long>(width) * height);
+    return s.scale();
 }
 
 }  // namespace grrt_amd

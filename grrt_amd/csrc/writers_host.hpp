@@ -4,6 +4,7 @@
 // outputs, cli/main.cpp:398-434).
 #pragma once
 #include <cstdint>
+#include <cstdio>
 #include <string>
 #include <vector>
 
@@ -46,13 +47,43 @@ private:
     std::vector<unsigned char> line_;
 };
 
-// Radiance RGBE image (new-style run-length scanlines), top row first
-// ("-Y height +X width").  rgba: width*height*4 floats; each RGB is scaled
-// by `scale` before encoding.
+// Row-streamed Radiance RGBE image (new-style run-length scanlines), top
+// row first ("-Y height +X width"): the header is written at open and rows
+// are appended in order as they arrive.  finish() writes any rows not yet
+// written as black, so the file always holds `height` rows.
+class RgbeWriter {
+public:
+    RgbeWriter(const std::string& path, int width, int height);
+    ~RgbeWriter();
+    RgbeWriter(const RgbeWriter&) = delete;
+    RgbeWriter& operator=(const RgbeWriter&) = delete;
+    // The next `rows` rows (rgba: rows*width*4 floats); each RGB is scaled by
+    // `scale` before encoding.
+    void write_rows(const float* rgba, int rows, float scale);
+    void finish();
+    int rows_written() const { return next_; }
+
+private:
+    std::FILE* f_ = nullptr;
+    int width_, height_, next_ = 0;
+    std::string path_;
+    std::vector<unsigned char> px_, out_;
+};
+
+// The whole frame through RgbeWriter.  rgba: width*height*4 floats.
 void write_rgbe(const std::string& path, const float* rgba, int width, int height, float scale);
 
-// Exposure scale of the CLI's normalised HDR: 0.18 over the log-average
-// luminance of the lit pixels (cli/main.cpp:410-425).
+// Exposure statistics of the CLI's normalised HDR (cli/main.cpp:410-425),
+// accumulated over pixels in index order: 0.18 over the log-average
+// luminance of the lit pixels.  Feeding a frame's rows in order gives the
+// same sum, term by term, as one pass over the frame.
+struct ExposureSum {
+    double log_sum = 0.0;
+    long long lit = 0;
+    void add(const float* rgba, long long n_pixels);
+    float scale() const;
+};
+
 float hdr_exposure_scale(const float* rgba, int width, int height);
 
 }  // namespace grrt_amd

@@ -305,6 +305,18 @@ GRRT_EXPORT int grrt_ext_write_fits_cube(const char* path, const void* cube, int
  * first), linear (normalised=0) or scaled by 0.18 over the log-average
  * luminance of the lit pixels (normalised=1), the CLI's two HDR outputs. */
 GRRT_EXPORT int grrt_ext_write_hdr(const char* path, const float* rgba, int width, int height, int normalised);
+/* Renders the RGB frame with the rows streamed to the CLI's two HDR outputs
+ * (replaces the render-then-write sequence of reference cli/main.cpp:386-434):
+ * bands of rows are rendered on the GPU, and each finished band is encoded
+ * and appended to path_linear while the next one renders.  path_scaled (may
+ * be NULL) then receives the frame scaled by 0.18 over the log-average
+ * luminance of the lit pixels.  framebuffer (may be NULL) receives the RGBA
+ * frame, equal to grrt_render's.  progress (may be NULL) is called per band on
+ * the caller's thread.  Returns 0; 1 when cancelled (path_linear is completed
+ * with black rows, the unrendered rows of framebuffer are zero, path_scaled is
+ * not written); -1 on error. */
+GRRT_EXPORT int grrt_ext_render_hdr(GRRTContext* ctx, const char* path_linear, const char* path_scaled,
+                                    float* framebuffer, grrt_progress_fn progress, void* user_data);
 
 #ifdef __cplusplus
 }
