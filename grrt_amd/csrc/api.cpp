@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <numeric>
 #include <chrono>
 #include <cmath>
 #include <cstddef>
@@ -226,6 +227,15 @@ void begin_call(GRRTContext* ctx) {
     ctx->progress_hi = 1.0f;
 }
 
+// Claim stride of the volumetric tracers within a row of L rays: the
+// smallest integer above L/8 that is coprime to L, so that the row's claim
+// order is a permutation and consecutive claims land about L/8 apart.
+long long spread_stride(long long L) {
+    long long m = L / 8 + 1;
+    while (std::gcd(m, L) != 1) ++m;
+    return m;
+}
+
 void frame_to_host(GRRTContext* ctx, float* host, const float* dev, size_t n) {
     if (ctx->host_stage_cap < n) {
         if (ctx->host_stage) (void)hipHostFree(ctx->host_stage);
@@ -236,13 +246,28 @@ void frame_to_host(GRRTContext* ctx, float* host, const float* dev, size_t n) {
         ctx->host_stage = static_cast<float*>(p);
         ctx->host_stage_cap = n;
     }
-    check(hipMemcpyAsync(ctx->host_stage, dev, n * sizeof(float), hipMemcpyDeviceToHost, ctx->stream), "copy frame");
-    check(hipStreamSynchronize(ctx->stream), "copy frame");
     if (!ctx->copy_pool) {
         const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
         ctx->copy_pool = std::make_unique<grrt_amd::CopyPool>(static_cast<int>(std::clamp(hw / 2, 1u, 8u)) - 1);
     }
-    ctx->copy_pool->copy(host, ctx->host_stage, n * sizeof(float));
+    // The DMA copy into pinned staging runs in chunks; the host threads copy
+    // chunk k into the caller's buffer while chunk k+1 is in flight.
+    constexpr int kChunks = sizeof(ctx->copy_ev) / sizeof(ctx->copy_ev[0]);
+    const size_t bytes = n * sizeof(float);
+    const size_t chunk = ((bytes + kChunks - 1) / kChunks + 4095) & ~static_cast<size_t>(4095);
+    char* const stage = reinterpret_cast<char*>(ctx->host_stage);
+    const char* const src = reinterpret_cast<const char*>(dev);
+    int issued = 0;
+    for (size_t off = 0; off < bytes; off += chunk, ++issued) {
+        check(hipMemcpyAsync(stage + off, src + off, std::min(chunk, bytes - off), hipMemcpyDeviceToHost, ctx->stream),
+              "copy frame");
+        check(hipEventRecord(ctx->copy_ev[issued], ctx->stream), "hipEventRecord");
+    }
+    for (int k = 0; k < issued; ++k) {
+        const size_t off = static_cast<size_t>(k) * chunk;
+        check(hipEventSynchronize(ctx->copy_ev[k]), "copy frame");
+        ctx->copy_pool->copy(reinterpret_cast<char*>(host) + off, stage + off, std::min(chunk, bytes - off));
+    }
 }
 
 enum class Output { kRGB, kSpectralF64, kSpectralF32 };
@@ -307,6 +332,7 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
     if (vol) {
         P.vol = ctx->vol;
         P.nbins = nb;
+        P.row_stride = spread_stride(static_cast<long long>(tw) * nS);
         P.far_f32 = ctx->far_field_f32 ? 1 : 0;
         if (spectral) {
             std::vector<double> lg(nb);
@@ -632,6 +658,7 @@ GRRTContext* grrt_create(const GRRTParams* params) {
         check(hipStreamCreateWithFlags(&ctx->stream2, hipStreamNonBlocking), "hipStreamCreate");
         for (auto& e : ctx->ev) check(hipEventCreate(&e), "hipEventCreate");
         for (auto& e : ctx->band_ev) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
+        for (auto& e : ctx->copy_ev) check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
         void* shared = nullptr;
         check(hipHostMalloc(&shared, sizeof(grrt_amd::HostShared), hipHostMallocMapped | hipHostMallocCoherent),
               "hipHostMalloc");
@@ -707,6 +734,8 @@ void grrt_destroy(GRRTContext* ctx) {
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto& e : ctx->band_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& e : ctx->copy_ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream2) (void)hipStreamDestroy(ctx->stream2);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);

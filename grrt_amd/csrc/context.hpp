@@ -76,6 +76,7 @@ struct GRRTContext {
     hipStream_t stream2 = nullptr;           // second stream (default priority): surrogate fallback traces
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     hipEvent_t band_ev[4] = {nullptr, nullptr, nullptr, nullptr};   // cross-stream ordering, no timing
+    hipEvent_t copy_ev[4] = {nullptr, nullptr, nullptr, nullptr};   // chunks of a frame copy, no timing
 
     // Scene tables in device memory.
     grrt_amd::CameraFrame cam{};

@@ -35,11 +35,11 @@ struct Vec5 {
 // FP64 operation tally (add/sub/mul = 1, fma = 2, reciprocal = 1):
 //   sincos 36, trig products 3, Sigma/Delta 5, reciprocal block 5,
 //   dt 10, dphi 8, dr/dtheta 3, u/v 8, dp_r 20, dp_theta 13  => 111.
-__device__ __forceinline__ Vec5 kerr_rhs(const KerrK& K, double P0, double P3, const Vec5& y,
-                                          double& dt_out) {
-    const double r = y.v[0], th = y.v[1], pr = y.v[3], pth = y.v[4];
-    double s, c;
-    sincos_cephes(th, s, c);
+// kerr_rhs_sc takes sin and cos of y.v[1] from the caller (sincos_cephes of
+// that same angle), for callers that need them as well.
+__device__ __forceinline__ Vec5 kerr_rhs_sc(const KerrK& K, double P0, double P3, const Vec5& y,
+                                             double& dt_out, double s, double c) {
+    const double r = y.v[0], pr = y.v[3], pth = y.v[4];
     // Pole guard |sin theta| >= 1e-10 (kerr.h:41); a sin of exactly -0 keeps its sign.
     s = __builtin_copysign(__builtin_fmax(__builtin_fabs(s), 1e-10), s);
     const double s2 = s * s;
@@ -75,6 +75,13 @@ __device__ __forceinline__ Vec5 kerr_rhs(const KerrK& K, double P0, double P3, c
     d.v[4] = sc * __builtin_fma(bt, iS2, Del * dph2);
     dt_out = dt;
     return d;
+}
+
+__device__ __forceinline__ Vec5 kerr_rhs(const KerrK& K, double P0, double P3, const Vec5& y,
+                                          double& dt_out) {
+    double s, c;
+    sincos_cephes(y.v[1], s, c);
+    return kerr_rhs_sc(K, P0, P3, y, dt_out, s, c);
 }
 
 }  // namespace grrt_amd

@@ -145,6 +145,9 @@ struct TraceParams {
     // Volumetric disk (the volumetric tracer only).
     VolDev vol;
     int nbins;                          // 3 for RGB, else spectral bins
+    // Claim order within a row: position p of the row's rays is ray
+    // (p * row_stride) mod row_rays, row_stride coprime to row_rays.
+    long long row_stride;
     int far_f32;                        // FP32 transfer path: FP32 geodesic steps outside r = 15 M
     int integrator;                     // 0: adaptive Dormand-Prince (the reference tracer's); 1: step-doubling RK4
     const double* nu_obs;               // observer-frame frequencies [Hz]

@@ -68,8 +68,10 @@ __device__ __forceinline__ double step_factor_v(double x) {
 // counts attempts.  k1 is the right-hand side at s on entry and at the new
 // state on return: the accepted stage k7 is evaluated at y5, which becomes
 // the state (first same as last), so the next step and the near-disk step
-// clamp reuse it instead of evaluating it again.
-__device__ double adaptive_dp45(const KerrK& K, GState& s, Vec5& k1, double dl, double tol, int& attempts) {
+// clamp reuse it instead of evaluating it again.  sn, cn: sin and cos of the
+// new state's theta, from the same k7 evaluation.
+__device__ double adaptive_dp45(const KerrK& K, GState& s, Vec5& k1, double dl, double tol, int& attempts,
+                                double& sn, double& cn) {
     double dtmp;
     for (int retry = 0; retry <= 20; ++retry) {
         const double h = dl;
@@ -98,7 +100,9 @@ __device__ double adaptive_dp45(const KerrK& K, GState& s, Vec5& k1, double dl, 
             e.v[i] = __builtin_fma(k6.v[i], h * E6, __builtin_fma(k5.v[i], h * E5,
                          __builtin_fma(k4.v[i], h * E4, __builtin_fma(k3.v[i], h * E3, k1.v[i] * (h * E1)))));
         }
-        k7 = kerr_rhs(K, s.P0, s.P3, y5, dtmp);
+        double s7, c7;
+        sincos_cephes(y5.v[1], s7, c7);
+        k7 = kerr_rhs_sc(K, s.P0, s.P3, y5, dtmp, s7, c7);
         double err = 0.0;
         for (int i = 0; i < 5; ++i)
             err = __builtin_fmax(err, __builtin_fabs(__builtin_fma(k7.v[i], h * E7, e.v[i])) *
@@ -107,6 +111,8 @@ __device__ double adaptive_dp45(const KerrK& K, GState& s, Vec5& k1, double dl, 
         if (retry == 20) {
             s.y = y5;
             k1 = k7;
+            sn = s7;
+            cn = c7;
             return h;
         }
         if (err <= tol) {
@@ -114,12 +120,16 @@ __device__ double adaptive_dp45(const KerrK& K, GState& s, Vec5& k1, double dl, 
             f = sclamp(f, 1.0, 5.0);
             s.y = y5;
             k1 = k7;
+            sn = s7;
+            cn = c7;
             return smin(h * f, 5.0 * smax(__builtin_fabs(y5.v[0]), 1.0));
         }
         dl = h * sclamp(step_factor_v(tol / err), 0.2, 1.0);
         if (dl < 1e-6) {
             s.y = y5;
             k1 = k7;
+            sn = s7;
+            cn = c7;
             return smin(1e-6, 5.0 * smax(__builtin_fabs(y5.v[0]), 1.0));
         }
     }
@@ -399,17 +409,18 @@ struct MarchOut {
 // state.  RGB mode: channels at 450/550/650 nm, step control on 550 nm and
 // an opacity exit; spectral mode: step control on bin N/2 and an exit once
 // every bin is past tau = 10.  Each lane updates the bins of its slots.
+// sth, cth: sin and cos of st's theta on entry, and of the exit state's on
+// return (every step evaluates them for the sample height; the next step's
+// first right-hand side takes them instead of evaluating them again).
 template <class RT, int kKind>
 __device__ MarchOut raymarch(const TraceParams& P, const VolDev& V, const VolRadial& R, const int* perm,
                              const double* nu_tab, const double* lnu_tab, const KerrK& K, GState& st,
-                             BinsOf<RT, kKind>& b, double lnu_rgb) {
+                             BinsOf<RT, kKind>& b, double lnu_rgb, double& sth, double& cth) {
     constexpr bool kSpectral = kKind != kBinsRGB;
     const int med = kSpectral ? P.nbins / 2 : 1;
     const double ut_obs = 1.0 / sqrt(1.0 - 2.0 / P.view.r_obs);
     MarchOut mo{0, 0};
     double r = st.y.v[0];
-    double sth, cth;
-    sincos_cephes(st.y.v[1], sth, cth);
     const double z0 = r * cth;
     const double H0 = vol_H(V, R, r);
     bool been_inside = vol_inside(V, R, r, z0);
@@ -422,7 +433,8 @@ __device__ MarchOut raymarch(const TraceParams& P, const VolDev& V, const VolRad
         ext = kSpectral ? (b.tau(j) < ext ? b.tau(j) : ext) : (b.T(j) > ext ? b.T(j) : ext);
     }
     for (int n = 0; n < 4096; ++n) {
-        const GState ns = rk4_step(K, st, ds);
+        double dtmp;
+        const GState ns = rk4_step_from(K, st, kerr_rhs_sc(K, st.P0, st.P3, st.y, dtmp, sth, cth), ds);
         ++mo.steps;
         r = ns.y.v[0];
         const double th = ns.y.v[1], ph = ns.y.v[2];
@@ -494,13 +506,18 @@ __device__ MarchOut raymarch(const TraceParams& P, const VolDev& V, const VolRad
 // Launch order -> ray index (row-major samples of the window): whole rows
 // centre-out, the middle row first and then alternately the rows above and
 // below it, so the rays that cross the disk, the longest ones, start first
-// instead of holding the launch open at its end.
-__device__ __forceinline__ long long centre_out_ray(const ViewParams& V, long long q) {
+// instead of holding the launch open at its end.  Within a row, claim
+// position p takes ray (p * stride) mod row_rays (stride coprime to the row
+// length, about an eighth of it), so the lane groups of one wavefront, which
+// claim consecutive positions, trace rays spread across the row.  Long rays
+// cluster in a few neighbouring pixels; with row-major claims several of
+// them shared a wavefront and ran their divergent paths one after another.
+__device__ __forceinline__ long long centre_out_ray(const ViewParams& V, long long q, long long stride) {
     const long long row_rays = static_cast<long long>(V.tw) * V.nS;
     const long long k = q / row_rays;
     const long long mid = V.th / 2, d = (k + 1) / 2;
     const long long row = (k & 1) ? mid - d : mid + d;
-    return row * row_rays + (q - k * row_rays);
+    return row * row_rays + ((q - k * row_rays) * stride) % row_rays;
 }
 
 }  // namespace
@@ -569,7 +586,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
         qv = __shfl(qv, leader);
         stop = __shfl(stop, leader);
         if (qv >= n_queue || stop) break;
-        const long long id = centre_out_ray(Vw, static_cast<long long>(qv));
+        const long long id = centre_out_ray(Vw, static_cast<long long>(qv), P.row_stride);
         GState s;
         SampleId sid;
         decode_ray(Vw, id, sid);      // row-major order: every id is a sample of the window
@@ -584,19 +601,21 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
         double dl = 0.01 * Vw.r_obs;
         int term = 2, steps = 0, attempts = 0, entries = 0, emitted = 0, march_steps = 0;
         double dtmp;
-        Vec5 k1 = kerr_rhs(K, s.P0, s.P3, s.y, dtmp);   // right-hand side at s
+        double sth_s, cth_s;                              // sin, cos of s's theta
+        sincos_cephes(s.y.v[1], sth_s, cth_s);
+        Vec5 k1 = kerr_rhs_sc(K, s.P0, s.P3, s.y, dtmp, sth_s, cth_s);   // right-hand side at s
         bool turned = false;                              // p_theta changed sign in the last step
         for (int step = 0; step < P.max_steps; ++step) {
             const double r = s.y.v[0];
             if (r < Vw.r_term) { term = 0; break; }
             if (r > P.r_escape) { term = 1; break; }
             const GState prev = s;
+            const double cth_p = cth_s;
             bool disk_zone = false;
             {
                 // near-disk step clamp from dz/dlambda at s (k1 is the
                 // derivative the reference evaluates there)
-                double sth, cth;
-                sincos_cephes(s.y.v[1], sth, cth);
+                const double sth = sth_s, cth = cth_s;
                 const double z = r * cth;
                 if (r >= V.r_horizon * 0.9 && r <= V.r_outer * 1.5) {
                     const double H = vol_H(V, R, r), zm = vol_zmax(V, R, r);
@@ -616,12 +635,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
             const double pth_before = s.y.v[4];
             if (kFarFieldF32<RT> && P.far_f32 && r > kFarField * Vw.M && s.y.v[3] > 0.0 && !disk_zone && !turned) {
                 dl = adaptive_dp45_f32(K, s, dl, attempts);
-                k1 = kerr_rhs(K, s.P0, s.P3, s.y, dtmp);
+                sincos_cephes(s.y.v[1], sth_s, cth_s);
+                k1 = kerr_rhs_sc(K, s.P0, s.P3, s.y, dtmp, sth_s, cth_s);
             } else if (P.integrator == 1) {
                 dl = adaptive_rk4_doubling(K, s, dl, P.tol, attempts);
-                k1 = kerr_rhs(K, s.P0, s.P3, s.y, dtmp);
+                sincos_cephes(s.y.v[1], sth_s, cth_s);
+                k1 = kerr_rhs_sc(K, s.P0, s.P3, s.y, dtmp, sth_s, cth_s);
             } else {
-                dl = adaptive_dp45(K, s, k1, dl, P.tol, attempts);
+                dl = adaptive_dp45(K, s, k1, dl, P.tol, attempts, sth_s, cth_s);
             }
             turned = (pth_before < 0.0) != (s.y.v[4] < 0.0);
             ++steps;
@@ -637,10 +658,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
                 const double tp = prev.y.v[1], tn = s.y.v[1];
                 const double dp = tp - half_pi, dn = tn - half_pi;
                 const double rn = s.y.v[0], rp = prev.y.v[0];
-                double sn, cn, sp_, cp_;
-                sincos_cephes(tn, sn, cn);
-                sincos_cephes(tp, sp_, cp_);
-                const double zn = rn * cn, zp = rp * cp_;
+                const double zn = rn * cth_s, zp = rp * cth_p;
                 const bool crossed = (dp * dn < 0.0) && __builtin_fabs(dp - dn) > 1e-12;
                 const bool inside_now = vol_inside(V, R, rn, zn);
                 const bool near = (__builtin_fabs(zn) < vol_zmax(V, R, rn) + 2.0 * vol_H(V, R, rn) ||
@@ -650,12 +668,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
                     const double rlo = smin(rp, rn), rhi = smax(rp, rn);
                     if (rhi >= V.r_horizon && rlo <= V.r_outer && rp >= V.r_horizon * 0.9 && rp <= V.r_outer * 1.5) {
                         GState entry = prev;
-                        const MarchOut mo = raymarch<RT, kKind>(P, V, R, s_perm, nu_tab, lnu_tab, K, entry, b, lnu_rgb);
+                        sincos_cephes(entry.y.v[1], sth_s, cth_s);
+                        const MarchOut mo = raymarch<RT, kKind>(P, V, R, s_perm, nu_tab, lnu_tab, K, entry, b, lnu_rgb,
+                                                                sth_s, cth_s);
                         emitted |= mo.emitted;
                         march_steps += mo.steps;
                         ++entries;
                         s = entry;
-                        k1 = kerr_rhs(K, s.P0, s.P3, s.y, dtmp);
+                        k1 = kerr_rhs_sc(K, s.P0, s.P3, s.y, dtmp, sth_s, cth_s);
                         continue;
                     }
                 }
