@@ -39,6 +39,8 @@ BF16_PEAK_TFLOPS = 2500.0         # MI355X dense BF16 MFMA peak
 SURROGATE_FLOPS_PER_RAY = 2.0 * (5 * 256 + 3 * 256 * 256 + 256 * 9)
 # Surrogate gates: class entropy (nats), redshift consistency (ln g), judged share.
 SG_ENTROPY, SG_LNG_TOL, SG_JUDGE = 0.1, 0.005, 0.02
+# Critical-curve gate (the library's default): rays with |critical score| below it fall back.
+SG_CRITICAL = 0.03
 # HBM bytes moved by one plain-frame trace launch: rocprofv3 --pmc FETCH_SIZE
 # (10 174 KiB, x2 by a copy-kernel calibration) + WRITE_SIZE (118 150 KiB,
 # x1), i.e. the 25-byte per-sample outputs, table refills and the few
@@ -140,6 +142,7 @@ def surrogate_section(grrt, ctx, plain_value, steps, warmup, sg_steps, big_frame
     import numpy as np
     t0 = time.perf_counter()
     ctx.surrogate_init(1)
+    ctx.set_critical_gate(SG_CRITICAL)
     ctx.surrogate_generate(1 << 24, seed=7)
     ctx.surrogate_generate(1 << 16, seed=8, validation=True)
     gen_s = time.perf_counter() - t0
@@ -186,7 +189,8 @@ def surrogate_section(grrt, ctx, plain_value, steps, warmup, sg_steps, big_frame
         "median_lit_pixel_flux_rel_err_vs_plain": round(med_all, 5),
         "judge_class_agreement": round(last["judge_class_agreement"], 6),
         "judge_median_rel_err": round(last["judge_median_rel_err"], 5),
-        "gates": {"entropy_max_nats": SG_ENTROPY, "lng_tol": SG_LNG_TOL, "judge_fraction": SG_JUDGE},
+        "gates": {"entropy_max_nats": SG_ENTROPY, "lng_tol": SG_LNG_TOL, "judge_fraction": SG_JUDGE,
+                  "critical_score_max": SG_CRITICAL},
         "training": {"rays": 1 << 24, "steps": sg_steps, "batch": 16384, "generate_s": round(gen_s, 2),
                      "train_s": round(train_s, 2), "val_accuracy": round(ev["accuracy"], 5),
                      "val_rmse_ln_r": round(ev["rmse_lnr"], 5), "val_rmse_ln_g": round(ev["rmse_lng"], 5)},
@@ -206,6 +210,7 @@ def surrogate_section(grrt, ctx, plain_value, steps, warmup, sg_steps, big_frame
         with grrt.Context(pbig) as cbig:
             cbig.set_sampling(grrt.SAMPLING_SEEDED, SPP, SEED)
             cbig.surrogate_init(1)
+            cbig.set_critical_gate(SG_CRITICAL)
             cbig.surrogate_params(ctx.surrogate_params())
             plain_big = cbig.render()
             plain_big_cls = cbig.classes()

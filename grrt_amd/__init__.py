@@ -180,6 +180,7 @@ API = [
     ("grrt_ext_surrogate_forward", C.c_int, [_P, _P, C.c_int64, _P]),
     ("grrt_ext_surrogate_gradients", C.c_int, [_P, _P, _P, C.c_int64, _P, _P, C.POINTER(C.c_double)]),
     ("grrt_ext_surrogate_dataset", C.c_int, [_P, C.c_int, _P, _P, C.c_int64]),
+    ("grrt_ext_surrogate_set_critical_gate", C.c_int, [_P, C.c_double]),
     ("grrt_ext_render_surrogate", C.c_int, [_P, _P, C.c_int, C.c_double, C.c_double, C.c_double,
                                             C.POINTER(GRRTSurrogateStats)]),
     ("grrt_ext_surrogate_infer_raw", C.c_int, [_P, _P, C.c_int64, _P]),
@@ -457,6 +458,11 @@ class Context:
         self._check(self._L.grrt_ext_surrogate_dataset(self._h, 1 if validation else 0, x.ctypes.data,
                                                         t.ctypes.data, n), "surrogate_dataset")
         return x, t
+
+    def set_critical_gate(self, score_max: float) -> None:
+        """grrt_ext_surrogate_set_critical_gate: rays with |critical score|
+        below score_max fall back to the integrator (0: off)."""
+        self._check(self._L.grrt_ext_surrogate_set_critical_gate(self._h, score_max), "set_critical_gate")
 
     def render_surrogate(self, entropy_max=0.1, judge_fraction=0.02, lng_tol=0.0, d_ptr: int | None = None,
                          device: bool = False):

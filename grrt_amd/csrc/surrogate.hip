@@ -365,6 +365,46 @@ __device__ __forceinline__ void decode_view_ray(const ViewParams& V, long long q
 
 // One 32-neuron block of a layer from a weight window: the bias, then the
 // 16 k-step MFMA chain with the window's reads issued three k-steps ahead.
+// Distance of a null ray from the critical curve (the edge of the shadow).
+// With E = -p_t, L = p_phi, b = L/E and q = Q/E^2 (Carter's constant
+// Q = p_theta^2 + cos^2 theta (L^2 / sin^2 theta - a^2 E^2), conserved, so
+// the camera's covector fixes it), the radial potential of Kerr null
+// geodesics in units of E^2 is
+//   R(r) = (r^2 + a^2 - a b)^2 - Delta(r) k,  k = q + (b - a)^2.
+// A ray on the critical curve has a double root of R at a spherical photon
+// orbit, whose radii run from the prograde to the retrograde equatorial
+// photon orbit (r0..r1).  The score is the minimum of R(r) / r^4 over that
+// range: 0 on the critical curve, > 0 for rays with no turning point there
+// (captured), < 0 for rays that turn outside it.  In x = 1/r,
+//   R / r^4 = 1 + c2 x^2 + c1 x^3 + c0 x^4,
+//   c2 = 2 A - k, c1 = 2 M k, c0 = A^2 - a^2 k, A = a^2 - a b,
+// a quartic whose stationary points other than x = 0 solve
+// 4 c0 x^2 + 3 c1 x + 2 c2 = 0: the minimum is the least of the quartic at
+// the range's ends and at those roots that fall inside it.
+// sth, cth: sin and cos of the ray's polar angle theta.
+__device__ double critical_score(double a, double M, double P0, double P3, double pth, double sth, double cth,
+                                 double r0, double r1) {
+    const double b = -P3 / P0;
+    const double s = fmax(fabs(sth), 1e-10), c = cth;
+    const double pe = pth / P0, a2 = a * a;
+    const double k = pe * pe + c * c * (b * b / (s * s) - a2) + (b - a) * (b - a);
+    const double A = a2 - a * b;
+    const double c2 = 2.0 * A - k, c1 = 2.0 * M * k, c0 = A * A - a2 * k;
+    auto f = [&](double x) { return 1.0 + x * x * (c2 + x * (c1 + x * c0)); };
+    const double xl = 1.0 / r1, xh = 1.0 / r0;
+    double lo = fmin(f(xl), f(xh));
+    const double qa = 4.0 * c0, qb = 3.0 * c1, qc = 2.0 * c2;
+    const double disc = qb * qb - 4.0 * qa * qc;
+    if (disc >= 0.0) {
+        // the two roots without cancellation: qq / qa and qc / qq
+        const double qq = -0.5 * (qb + copysign(sqrt(disc), qb));
+        const double xs[2] = {qa != 0.0 ? qq / qa : -1.0, qq != 0.0 ? qc / qq : -1.0};
+        for (double x : xs)
+            if (x >= xl && x <= xh) lo = fmin(lo, f(x));
+    }
+    return lo;
+}
+
 __device__ __forceinline__ f32x16 window_block(const bf16x8* W, const bf16x8 (&in)[16], const float* bias, int mt,
                                                int g, int lane) {
     f32x16 a = bias_tile(bias, mt, g, kHid);
@@ -583,20 +623,40 @@ __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
         // the conserved p_t, p_phi of its camera ray (src/accretion_disk.cpp:
         // 113-130): shade with that g, and send the ray to the integrator when
         // the network's own ln g disagrees with it by more than lng_tol.
+        // Rays near the critical curve (the shadow's edge) go to the
+        // integrator whatever the network predicts.
         double rc = 0.0, g_exact = 0.0;
-        bool inconsistent = false;
-        if (valid && cls == GRRT_CLASS_DISK) {
+        bool inconsistent = false, near_critical = false;
+        if (valid && (cls == GRRT_CLASS_DISK || IA.crit_max > 0.0)) {
             SampleId d;
             decode_view_ray(V, i, d);
-            double P0, P3;
-            Vec5 y0;
-            camera_ray(V, d, P0, P3, y0);
-            rc = exp(static_cast<double>(y[3]));
-            g_exact = disk_redshift(V, rc, P0, P3);
-            inconsistent = IA.lng_tol > 0.0 &&
-                           !(fabs(log(fmax(g_exact, 1e-300)) - static_cast<double>(y[4])) <= IA.lng_tol);
+            if (cls == GRRT_CLASS_DISK) {
+                double P0, P3;
+                Vec5 y0;
+                camera_ray(V, d, P0, P3, y0);
+                rc = exp(static_cast<double>(y[3]));
+                g_exact = disk_redshift(V, rc, P0, P3);
+                inconsistent = IA.lng_tol > 0.0 &&
+                               !(fabs(log(fmax(g_exact, 1e-300)) - static_cast<double>(y[4])) <= IA.lng_tol);
+            }
+            if (IA.crit_max > 0.0) {
+                // the camera covector of camera_ray, its two pixel angles'
+                // sin/cos by the branch-free Cephes form (the gate needs
+                // ~1e-9, not the integrator's last bit)
+                double px, py, sa, ca_, sb, cb;
+                sample_position(V, d, px, py);
+                sincos_cephes((px - V.W / 2.0) * V.fov / V.W, sa, ca_);
+                sincos_cephes((py - V.H / 2.0) * V.fov / V.W, sb, cb);
+                const double ca = cb * sa, cc = cb * ca_;
+                const double p0 = -V.e0t + (-ca * V.e1[0] - sb * V.e2[0] + cc * V.e3[0]);
+                const double p2 = -ca * V.e1[2] - sb * V.e2[2] + cc * V.e3[2];
+                const double p3 = -ca * V.e1[3] - sb * V.e2[3] + cc * V.e3[3];
+                const double P0 = V.g00 * p0 + V.g03 * p3, P3 = V.g03 * p0 + V.g33 * p3;
+                near_critical = fabs(critical_score(V.a, V.M, P0, P3, V.g22 * p2, IA.crit_sth, IA.crit_cth,
+                                                    IA.crit_r0, IA.crit_r1)) < IA.crit_max;
+            }
         }
-        const bool fb = valid && (static_cast<double>(H) > IA.entropy_max || inconsistent);
+        const bool fb = valid && (static_cast<double>(H) > IA.entropy_max || inconsistent || near_critical);
         const long long pf = wave_append(fb, &IA.counts[0]);
         if (fb) IA.fb_list[pf] = i;
         const bool shade = valid && !fb;

@@ -103,6 +103,11 @@ struct InferArgs {
     const TraceParams* P;               // the frame's trace parameters (device memory)
     double entropy_max;                 // nats; above it the ray falls back
     double lng_tol;                     // |ln g_net - ln g(r_net)| above it falls back (<= 0: off)
+    // Critical-curve gate: a ray whose |critical_score| is below crit_max
+    // falls back (<= 0: off); crit_r0, crit_r1: the view's prograde and
+    // retrograde equatorial photon-orbit radii, the range of the spherical
+    // photon orbits; crit_sth, crit_cth: sin and cos of the observer's theta.
+    double crit_max, crit_r0, crit_r1, crit_sth, crit_cth;
     uint32_t judge_threshold;           // hash < threshold (of 2^32) selects a judged ray
     uint32_t frame_seed;
     long long ray_base;                 // first ray of the launch (rays ray_base .. ray_base + mlp.n)

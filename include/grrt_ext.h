@@ -261,7 +261,8 @@ GRRT_EXPORT int grrt_ext_surrogate_gradients(GRRTContext* ctx, const float* x, c
 /* Copies rows [0, n) of the training (validation=0) or validation set. */
 GRRT_EXPORT int grrt_ext_surrogate_dataset(GRRTContext* ctx, int validation, float* x, float* t, int64_t n);
 /* Surrogate frame of the context's view: every sample is shaded from the
- * network unless its class entropy exceeds entropy_max (nats) or, for a
+ * network unless its class entropy exceeds entropy_max (nats), its ray lies
+ * near the critical curve (grrt_ext_surrogate_set_critical_gate) or, for a
  * predicted disk hit, the network's ln g differs by more than lng_tol from
  * the redshift that the predicted radius and the ray's conserved p_t, p_phi
  * fix (lng_tol <= 0: no such check); those samples are traced by the FP64
@@ -270,6 +271,14 @@ GRRT_EXPORT int grrt_ext_surrogate_dataset(GRRTContext* ctx, int validation, flo
  * compared (stats).  out: RGBA float32 as grrt_render (host memory, or
  * device memory when device_out != 0; NULL with device_out != 0 keeps the
  * frame in the context's device memory). */
+/* Critical-curve gate of grrt_ext_render_surrogate: a sample whose camera
+ * ray lies near the critical curve (the edge of the black hole's shadow) is
+ * traced by the FP64 integrator whatever the network predicts.  The test is
+ * |s| < score_max with s the minimum of R(r) / (E^2 r^4) over the spherical
+ * photon-orbit radii, R the Kerr radial potential of the ray's conserved
+ * E, L and Carter constant (0 on the critical curve).  Default 0.03;
+ * 0 turns the gate off. */
+GRRT_EXPORT int grrt_ext_surrogate_set_critical_gate(GRRTContext* ctx, double score_max);
 GRRT_EXPORT int grrt_ext_render_surrogate(GRRTContext* ctx, float* out, int device_out, double entropy_max,
                                           double lng_tol, double judge_fraction, GRRTSurrogateStats* stats);
 /* The surrogate frame's network pass itself, writing the network's 9 raw

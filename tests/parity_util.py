@@ -22,3 +22,34 @@ def median_rel_err(a: np.ndarray, b: np.ndarray, floor: float = 0.0) -> float:
     if not sel.any():
         return 0.0
     return float(np.median(np.abs(a[sel] - b[sel]) / np.abs(b[sel])))
+
+
+def photon_orbit_radii(a: float, M: float = 1.0):
+    """Prograde and retrograde equatorial photon-orbit radii of Kerr
+    (Bardeen, Press & Teukolsky 1972), smaller first."""
+    x = min(max(a / M, -1.0), 1.0)
+    rp = 2 * M * (1 + np.cos(2.0 / 3.0 * np.arccos(-x)))
+    rm = 2 * M * (1 + np.cos(2.0 / 3.0 * np.arccos(x)))
+    return min(rp, rm), max(rp, rm)
+
+
+def critical_score(a, M, p_t, p_th, p_ph, theta, n=None):
+    """Minimum over the spherical photon-orbit radii of R(r) / (E^2 r^4), R
+    the Kerr radial potential of a null ray with covariant momentum
+    (p_t, p_th, p_ph) at polar angle theta: 0 on the critical curve, > 0 for
+    rays with no turning point there, < 0 otherwise.  Dense sampling (n
+    radii, default 4001), no interpolation: a brute-force reference for the
+    library's closed form (the minimum of a quartic in 1/r)."""
+    p_t, p_th, p_ph, theta = (np.atleast_1d(np.asarray(v, np.float64)) for v in (p_t, p_th, p_ph, theta))
+    r0, r1 = photon_orbit_radii(a, M)
+    r = np.linspace(r0, r1, n or 4001)[None, :]
+    out = np.empty(len(p_t))
+    for i in range(0, len(p_t), 1024):          # bounded memory
+        sl = slice(i, i + 1024)
+        b = -p_ph[sl] / p_t[sl]
+        s = np.maximum(np.abs(np.sin(theta[sl])), 1e-10)
+        c = np.cos(theta[sl])
+        k = (p_th[sl] / p_t[sl]) ** 2 + c * c * (b * b / (s * s) - a * a) + (b - a) ** 2
+        u = r * r + a * a - a * b[:, None]
+        out[sl] = ((u * u - (r * r - 2 * M * r + a * a) * k[:, None]) / r ** 4).min(axis=1)
+    return out

@@ -267,6 +267,41 @@ def test_surrogate_frame_fallback_and_judge(grrt):
     assert np.array_equal(fb0, plain)
 
 
+def test_critical_gate_sends_near_critical_rays_to_the_integrator(grrt, oracle):
+    """With the entropy and redshift gates off, the fallback list of a frame
+    zoomed on the shadow is exactly the rays whose critical score (the CPU
+    restatement, brute-force minimum over 4001 radii) is below the gate,
+    apart from rays within 1e-4 of it (the sampling error of the
+    restatement, ~1e-6 here); the gate off lists no ray; the listed rays
+    carry the integrator's results bit for bit."""
+    from parity_util import critical_score
+    print("case: critical-curve gate, 128^2 a=0.998 fov 20 deg, gate 0.03", flush=True)
+    W, a, gate = 128, 0.998, 0.03
+    kw = dict(width=W, height=W, observer_theta=math.radians(80.0), fov=math.radians(20.0), stars=False)
+    with _ctx(grrt, **kw) as ctx:
+        ctx.set_sampling(grrt.SAMPLING_CENTER)
+        ctx.render()
+        pcls, prgb = ctx.classes(), ctx.samples()
+        ctx.surrogate_init(5)
+        ctx.set_critical_gate(0.0)
+        _, st0 = ctx.render_surrogate(entropy_max=1e30, judge_fraction=0.0)
+        ctx.set_critical_gate(gate)
+        _, st = ctx.render_surrogate(entropy_max=1e30, judge_fraction=0.0)
+        fbl, _ = ctx.surrogate_lists(st["fallback_rays"], 0)
+        scls, srgb = ctx.classes(), ctx.samples()
+    assert st0["fallback_rays"] == 0
+    sc = oracle.scene(width=W, height=W, spin_a=a, theta_obs=math.radians(80.0), fov=math.radians(20.0))
+    cam = np.array([oracle.camera_ray(sc, (k % W) + 0.5, (k // W) + 0.5) for k in range(W * W)])
+    score = np.abs(critical_score(a, 1.0, cam[:, 4], cam[:, 6], cam[:, 7], cam[:, 2]))
+    listed = np.zeros(W * W, bool)
+    listed[fbl] = True
+    must, may = score < gate - 1e-4, score < gate + 1e-4
+    print(f"listed {listed.sum()}, below the gate {np.sum(score < gate)}, firm {must.sum()}", flush=True)
+    assert must.sum() > 100
+    assert np.all(listed[must]) and not np.any(listed & ~may)
+    assert np.array_equal(scls[fbl], pcls[fbl]) and np.array_equal(srgb[fbl], prgb[fbl])
+
+
 def _disk_pixel_flux_err(s_fb, p_fb, s_cls, p_cls, nS):
     disk = (s_cls.reshape(-1, nS) == 2).any(1) | (p_cls.reshape(-1, nS) == 2).any(1)
     a = s_fb.reshape(-1, 4)[disk, :3].sum(1).astype(np.float64)

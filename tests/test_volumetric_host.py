@@ -9,11 +9,12 @@ charge-neutral root; the mu and electron-scattering tables, against the same
 map; the absorption table, against the point opacities; the Rosseland table,
 against adaptive quadrature where its band holds the dB/dT peak; the
 hydrogen bound-free edge, against Kramers' threshold cross-section; the
-vertical temperature profile, against its closed form.  The remaining
-frequency-dependent opacity terms (free-free, H-), the Rosseland table at
-other temperatures and the vertical density profile have no closed form and
-the reference cannot be built here: beyond the checks below they are
-"parity unpinned"."""
+vertical temperature profile, against its closed form (the vertical
+density profile and the radial flux are pinned in test_vol_structure_pins.py).
+The remaining frequency-dependent opacity terms (free-free, H-), the
+Rosseland table at other temperatures, the radial scale height and midplane
+density have no closed form and the reference cannot be built here: beyond
+the checks below they are "parity unpinned"."""
 import math
 
 import numpy as np
