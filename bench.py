@@ -334,16 +334,21 @@ def spectral_cube_section(grrt, reps: int, cpu: bool) -> dict:
     out = {"workload": "Kerr a=0.9 512^2 x 1 spp, theta=75deg, volumetric alpha=0.1 turb 0.4 2 oct seed 42, "
                        "32 bins 1e14-1e18 Hz, float32 cube",
            "context_setup_s": round(setup_s, 3)}
-    # Set-up of the disk tables: opacity tables on host threads against the
-    # GPU (grrt_create's path), then the radial and vertical structure on
+    # Set-up of the disk tables: opacity tables and vertical columns on host
+    # threads against the GPU (grrt_create's path), the radial structure on
     # the host in both; best of three builds each.
     builds = {}
     for dev in (False, True):
         runs = [grrt.volumetric_tables(p, device=dev) for _ in range(3)]
         builds["device" if dev else "host"] = (min(r["build_ms_opacity"] for r in runs),
-                                               min(r["build_ms_total"] for r in runs))
+                                               min(r["build_ms_total"] for r in runs),
+                                               min(r["build_ms_columns"] for r in runs),
+                                               min(r["build_ms_columns_kernel"] for r in runs))
     out["table_build_ms"] = {"opacity_host": round(builds["host"][0], 2),
                              "opacity_device": round(builds["device"][0], 2),
+                             "columns_host": round(builds["host"][2], 2),
+                             "columns_device": round(builds["device"][2], 2),
+                             "columns_device_kernel": round(builds["device"][3], 2),
                              "total_host": round(builds["host"][1], 2),
                              "total_device": round(builds["device"][1], 2),
                              # the library's own table build uses min(16, hardware threads)

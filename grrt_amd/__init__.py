@@ -196,6 +196,7 @@ API = [
     ("grrt_ext_set_vol_tuning", C.c_int, [_P, C.c_int]),
     ("grrt_ext_vol_build", _P, [C.POINTER(GRRTParams)]),
     ("grrt_ext_vol_build_device", _P, [C.POINTER(GRRTParams)]),
+    ("grrt_ext_vol_build_ex", _P, [C.POINTER(GRRTParams), C.c_int]),
     ("grrt_ext_vol_scalars", C.c_int, [_P, C.POINTER(C.c_double), C.c_int]),
     ("grrt_ext_vol_table", C.c_int64, [_P, C.c_char_p, C.POINTER(C.c_double), C.c_int64]),
     ("grrt_ext_vol_free", None, [_P]),
@@ -206,7 +207,8 @@ API = [
 
 VOL_SCALARS = ["r_isco", "r_horizon", "r_min", "r_outer", "taper_width", "E_isco", "L_isco",
                "rho_scale", "noise_scale", "turbulence", "octaves", "n_r", "n_z", "n_nu",
-               "log_nu_min", "log_nu_max", "M", "a", "t_peak", "build_ms_opacity", "build_ms_total"]
+               "log_nu_min", "log_nu_max", "M", "a", "t_peak", "build_ms_opacity", "build_ms_total",
+               "build_ms_columns", "build_ms_columns_kernel"]
 VOL_TABLES = ["H", "z_max", "rho_mid", "T_eff", "rho_profile", "T_profile", "kappa_abs",
               "kappa_es", "kappa_ross", "mu", "perm"]
 
@@ -265,13 +267,17 @@ def make_params(width=256, height=256, metric=METRIC_KERR, spin=0.998, mass=1.0,
     return p
 
 
-def volumetric_tables(params: GRRTParams, device: bool = False) -> dict:
+def volumetric_tables(params: GRRTParams, device: bool = False, device_columns: bool | None = None) -> dict:
     """Volumetric disk set-up: scalars and tables.  device=False builds on
-    host threads (no GPU needed); device=True computes the opacity tables on
-    the GPU, as grrt_create does."""
+    host threads (no GPU needed); device=True computes the opacity tables and
+    the vertical columns on the GPU, as grrt_create does (device_columns=False:
+    the columns on host threads over the GPU's opacity tables)."""
     L = lib()
-    fn = L.grrt_ext_vol_build_device if device else L.grrt_ext_vol_build
-    h = fn(C.byref(params))
+    if device and device_columns is False:
+        h = L.grrt_ext_vol_build_ex(C.byref(params), 1)
+    else:
+        fn = L.grrt_ext_vol_build_device if device else L.grrt_ext_vol_build
+        h = fn(C.byref(params))
     if not h:
         err = L.grrt_last_error()
         raise RuntimeError("volumetric table build failed: " + (err.decode() if err else "unknown"))

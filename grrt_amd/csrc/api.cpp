@@ -1095,8 +1095,13 @@ GRRTVolTables* grrt_ext_vol_build(const GRRTParams* params) {
     return nullptr;
 }
 
-GRRTVolTables* grrt_ext_vol_build_device(const GRRTParams* params) {
-    if (!params) { g_last_error = "grrt_ext_vol_build_device: params is null"; return nullptr; }
+GRRTVolTables* grrt_ext_vol_build_ex(const GRRTParams* params, int flags) {
+    if (!params) { g_last_error = "grrt_ext_vol_build_ex: params is null"; return nullptr; }
+    if ((flags & ~3) != 0 || flags == 2) {
+        g_last_error = "grrt_ext_vol_build_ex: flags must be 0, 1 or 3";
+        return nullptr;
+    }
+    if (flags == 0) return grrt_ext_vol_build(params);
     hipStream_t st = nullptr;
     GRRTVolTables* h = nullptr;
     try {
@@ -1105,7 +1110,7 @@ GRRTVolTables* grrt_ext_vol_build_device(const GRRTParams* params) {
         if (hipStreamCreate(&st) != hipSuccess) throw std::runtime_error("hipStreamCreate failed");
         const VolSetup vs = resolve_volumetric(*params);
         h = new GRRTVolTables();
-        grrt_amd::build_volumetric(vs.M, vs.a, vs.r_outer, vs.t_peak, vs.vp, h->t, &st);
+        grrt_amd::build_volumetric(vs.M, vs.a, vs.r_outer, vs.t_peak, vs.vp, h->t, &st, (flags & 2) != 0);
     } catch (const std::exception& e) {
         g_last_error = e.what();
         delete h;
@@ -1115,6 +1120,8 @@ GRRTVolTables* grrt_ext_vol_build_device(const GRRTParams* params) {
     return h;
 }
 
+GRRTVolTables* grrt_ext_vol_build_device(const GRRTParams* params) { return grrt_ext_vol_build_ex(params, 3); }
+
 int grrt_ext_vol_scalars(const GRRTVolTables* h, double* out, int cap) {
     if (!h || !out) return -1;
     const auto& t = h->t;
@@ -1122,7 +1129,7 @@ int grrt_ext_vol_scalars(const GRRTVolTables* h, double* out, int cap) {
                         t.rho_scale, t.noise_scale, t.prm.turbulence, static_cast<double>(t.prm.noise_octaves),
                         static_cast<double>(t.n_r), static_cast<double>(t.n_z), static_cast<double>(t.op.n_nu),
                         t.op.log_nu_min, t.op.log_nu_max, t.M, t.a, t.t_peak, t.build_ms_opacity,
-                        t.build_ms_total};
+                        t.build_ms_total, t.build_ms_columns, t.build_ms_columns_kernel};
     const int n = static_cast<int>(sizeof(v) / sizeof(v[0]));
     for (int i = 0; i < n && i < cap; ++i) out[i] = v[i];
     return n;

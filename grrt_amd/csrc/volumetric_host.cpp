@@ -10,6 +10,8 @@
 // include/grrt/math/constants.h:11-35); they are data and must match.
 #include "volumetric_host.hpp"
 
+#include "vol_column.hpp"
+
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -206,115 +208,69 @@ struct Builder {
                 if (V.r_min + (V.r_outer - V.r_min) * i / (N - 1) < V.r_isco) V.H[i] = H_isco;
     }
 
-    // Levermore & Pomraning (1981) flux limiter and Eddington factor.
-    static double eddington_factor(double R) {
-        const double lam = (2.0 + R) / (6.0 + 3.0 * R + R * R);
-        return lam + lam * lam * R * R;
-    }
-
     // src/volumetric_disk.cpp:505-777: per column, Eddington T(tau) and a
     // hydrostatic density profile with gas + flux-limited radiation
     // pressure, extending the column top until the density falls below
-    // 1e-10 of the midplane (up to 20 H), at most 8 passes.
-    void vertical() {
+    // 1e-10 of the midplane (up to 20 H), at most 8 passes.  The node steps
+    // are vol_column.hpp's, shared with the GPU build (device != nullptr),
+    // which gives the same tables bit for bit.
+    void vertical(const hipStream_t* device) {
         const int NR = V.n_r, NZ = V.n_z;
+        if (NZ < 2 || NZ > kColMaxZ) throw std::runtime_error("vertical columns: n_z out of range");
         V.z_max.assign(NR, 0.0);
         V.rho_profile.assign(static_cast<size_t>(NR) * NZ, 0.0);
         V.T_profile.assign(static_cast<size_t>(NR) * NZ, 0.0);
-        const double c2 = cgs::c * cgs::c;
-        auto column = [&](int ri) {
-            std::vector<double> rho(NZ), T(NZ), tau(NZ), E(NZ), f(NZ), mu(NZ), dcs2(NZ), dfE(NZ);
+        std::vector<ColumnIn> in(NR);
+        for (int ri = 0; ri < NR; ++ri) {
             const double r = V.r_min + (V.r_outer - V.r_min) * ri / (NR - 1);
-            const double H = V.H[ri], Te = V.T_eff[ri], rm = V.rho_mid[ri];
-            double* rp = &V.rho_profile[static_cast<size_t>(ri) * NZ];
-            double* tp = &V.T_profile[static_cast<size_t>(ri) * NZ];
-            if (H <= 0.0 || Te <= 0.0 || rm <= 0.0) {
-                V.z_max[ri] = 3.0 * H;
-                rp[0] = 1.0;
-                for (int z = 0; z < NZ; ++z) tp[z] = Te;
-                return;
-            }
             double Omz2 = omega_z_sq(r);
             if (r < V.r_isco || Omz2 <= 0.0) {
                 Omz2 = omega_z_sq(V.r_isco);
                 if (Omz2 <= 0.0) Omz2 = omega_orb(V.r_isco) * omega_orb(V.r_isco);
             }
-            const double kref = std::max(V.op.lookup_ross(1e-10, clamp_T(Te)) + V.op.lookup_es(1e-10, clamp_T(Te)), 1.0);
-            const double rho_ref = std::clamp(V.prm.tau_mid / (kref * 3.0 * H), 1e-18, 1e-6);
-            const double Te4 = Te * Te * Te * Te;
-            const double T_mid = std::pow(0.75 * Te4 * (V.prm.tau_mid + 2.0 / 3.0), 0.25);
-            double zmax = 3.0 * H;
-            const double floor_rho = 1e-15;
-            std::vector<double> prev(NZ, 1.0);
+            in[ri] = ColumnIn{V.H[ri], V.T_eff[ri], V.rho_mid[ri], Omz2};
+        }
+        const OpacityTables& o = V.op;
+        ColumnTables t{o.kappa_ross.data(), o.kappa_es.data(), o.mu.data(), o.n_rho, o.n_T,
+                       o.log_rho_min, o.log_rho_max, o.log_T_min, o.log_T_max};
+        if (device) {
+            columns_on_device(t, in, *device);
+            return;
+        }
+        auto column = [&](int ri) {
+#pragma clang fp contract(off)
+            double rho[kColMaxZ], T[kColMaxZ], tau[kColMaxZ], E[kColMaxZ], f[kColMaxZ], mu[kColMaxZ],
+                dcs2[kColMaxZ], dfE[kColMaxZ];
+            RhsTerms q0[kColMaxZ], qh[kColMaxZ], q1[kColMaxZ];
+            const ColumnIn& c = in[ri];
+            double* rp = &V.rho_profile[static_cast<size_t>(ri) * NZ];
+            double* tp = &V.T_profile[static_cast<size_t>(ri) * NZ];
+            if (c.H <= 0.0 || c.Te <= 0.0 || c.rm <= 0.0) {
+                V.z_max[ri] = 3.0 * c.H;
+                rp[0] = 1.0;
+                for (int z = 0; z < NZ; ++z) tp[z] = c.Te;
+                return;
+            }
+            const double rho_ref = col_rho_ref(t, c, V.prm.tau_mid);
+            const double Te4 = c.Te * c.Te * c.Te * c.Te;
+            const double T_mid = col_T(Te4, V.prm.tau_mid);
+            double zmax = 3.0 * c.H;
             for (int iter = 0; iter < 8; ++iter) {
                 const double dz = zmax / (NZ - 1);
-                std::fill(rho.begin(), rho.end(), 1.0);
-                std::fill(T.begin(), T.end(), T_mid);
-                std::fill(tau.begin(), tau.end(), 0.0);
-                for (int z = NZ - 2; z >= 0; --z) {
-                    const double rh = rho[z] * rho_ref, rn = rho[z + 1] * rho_ref;
-                    const double kh = V.op.lookup_ross(clamp_rho(rh), clamp_T(T[z])) + V.op.lookup_es(clamp_rho(rh), clamp_T(T[z]));
-                    const double kn = V.op.lookup_ross(clamp_rho(rn), clamp_T(T[z + 1])) + V.op.lookup_es(clamp_rho(rn), clamp_T(T[z + 1]));
-                    tau[z] = tau[z + 1] + 0.5 * (kh * rh + kn * rn) * dz;
-                }
-                for (int z = 0; z < NZ; ++z)
-                    T[z] = std::pow(std::max(0.75 * Te4 * (tau[z] + 2.0 / 3.0), 0.0), 0.25);
+                const double step = col_tau_step(t, rho_ref, T_mid, dz);
+                tau[NZ - 1] = 0.0;
+                for (int z = NZ - 2; z >= 0; --z) tau[z] = tau[z + 1] + step;
                 for (int z = 0; z < NZ; ++z) {
-                    E[z] = cgs::a_rad * T[z] * T[z] * T[z] * T[z];
-                    mu[z] = V.op.lookup_mu(clamp_rho(rho[z] * rho_ref), clamp_T(T[z]));
-                    if (mu[z] <= 0.0 || !std::isfinite(mu[z])) mu[z] = 0.6;
+                    T[z] = col_T(Te4, tau[z]);
+                    E[z] = col_E(T[z]);
+                    mu[z] = col_mu(t, rho_ref, T[z]);
                 }
-                for (int z = 0; z < NZ; ++z) {
-                    double dE = 0.0;
-                    if (z == NZ - 1) dE = (E[z] - E[z - 1]) / dz;
-                    else if (z > 0) dE = (E[z + 1] - E[z - 1]) / (2.0 * dz);
-                    const double rc = rho[z] * rho_ref;
-                    const double den = V.op.lookup_ross(clamp_rho(rc), clamp_T(T[z])) * rc * E[z];
-                    f[z] = eddington_factor(den < 1e-30 ? 1e30 : std::abs(dE) / den);
-                }
-                auto cs2 = [&](int z) { return cgs::k_B * T[z] / (mu[z] * cgs::m_p); };
-                for (int z = 0; z < NZ; ++z) {
-                    if (z == 0) { dcs2[z] = 0.0; dfE[z] = 0.0; continue; }
-                    if (z == NZ - 1) {
-                        dcs2[z] = (cs2(z) - cs2(z - 1)) / dz;
-                        dfE[z] = (f[z] * E[z] - f[z - 1] * E[z - 1]) / dz;
-                    } else {
-                        dcs2[z] = (cs2(z + 1) - cs2(z - 1)) / (2.0 * dz);
-                        dfE[z] = (f[z + 1] * E[z + 1] - f[z - 1] * E[z - 1]) / (2.0 * dz);
-                    }
-                }
-                // dRho/dz = [-rho Omz^2 z - rho d(cs^2)/dz - d(f E)/dz / rho_ref] / cs^2,
-                // geometric z with CGS thermodynamics converted by c^2.
-                auto rhs = [&](double ze, double re) {
-                    const double zf = ze / dz;
-                    const int idx = std::clamp(static_cast<int>(zf), 0, NZ - 2);
-                    const double t = zf - idx;
-                    const double cs = cgs::k_B * ((1.0 - t) * T[idx] + t * T[idx + 1]) /
-                                      (((1.0 - t) * mu[idx] + t * mu[idx + 1]) * cgs::m_p);
-                    const double dc = (1.0 - t) * dcs2[idx] + t * dcs2[idx + 1];
-                    const double df = (1.0 - t) * dfE[idx] + t * dfE[idx + 1];
-                    if (cs < 1e-30) return 0.0;
-                    return (-re * (Omz2 * ze) - re * (dc / c2) - df / (rho_ref * c2)) / std::max(cs / c2, 1e-30);
-                };
-                rho[0] = 1.0;
-                for (int z = 0; z < NZ - 1; ++z) {
-                    const double zh = z * dz, rh = rho[z];
-                    const double k1 = dz * rhs(zh, rh);
-                    const double k2 = dz * rhs(zh + 0.5 * dz, std::max(rh + 0.5 * k1, floor_rho));
-                    const double k3 = dz * rhs(zh + 0.5 * dz, std::max(rh + 0.5 * k2, floor_rho));
-                    const double k4 = dz * rhs(zh + dz, std::max(rh + k3, floor_rho));
-                    rho[z + 1] = std::max(rh + (k1 + 2.0 * k2 + 2.0 * k3 + k4) / 6.0, floor_rho);
-                }
-                if (rho[NZ - 1] > 1e-10 && zmax < 20.0 * H) {
-                    zmax = std::min(zmax + H, 20.0 * H);
-                    prev = rho;
-                    continue;
-                }
-                double dmax = 0.0;
-                for (int z = 0; z < NZ; ++z)
-                    if (prev[z] > floor_rho * 10.0) dmax = std::max(dmax, std::abs(rho[z] - prev[z]) / prev[z]);
-                prev = rho;
-                if (iter > 0 && dmax < 0.001) break;
+                for (int z = 0; z < NZ; ++z) f[z] = col_f(t, E, T, z, NZ, dz, rho_ref);
+                for (int z = 0; z < NZ; ++z) col_grads(T, mu, f, E, z, NZ, dz, dcs2[z], dfE[z]);
+                for (int z = 0; z < NZ - 1; ++z)
+                    col_step_terms(T, mu, dcs2, dfE, NZ, dz, c.Omz2, rho_ref, z, q0[z], qh[z], q1[z]);
+                col_rk4(q0, qh, q1, NZ, dz, rho);
+                if (!col_raise_top(rho, NZ, c.H, zmax)) break;
             }
             V.z_max[ri] = zmax;
             for (int z = 0; z < NZ; ++z) {
@@ -324,11 +280,56 @@ struct Builder {
         };
         const int nt = std::clamp(static_cast<int>(std::thread::hardware_concurrency()), 1, 16);
         std::vector<std::thread> pool;
-        for (int t = 0; t < nt; ++t)
-            pool.emplace_back([&, t] {
-                for (int ri = t; ri < NR; ri += nt) column(ri);
+        for (int th = 0; th < nt; ++th)
+            pool.emplace_back([&, th] {
+                for (int ri = th; ri < NR; ri += nt) column(ri);
             });
-        for (auto& th : pool) th.join();
+        for (auto& w : pool) w.join();
+    }
+
+    // The columns on the GPU: inputs and the three (rho, T) tables up,
+    // z_max and the two profiles back; stream-ordered, synchronised here.
+    void columns_on_device(const ColumnTables& th, const std::vector<ColumnIn>& in, hipStream_t st) {
+        const int NR = V.n_r, NZ = V.n_z;
+        const size_t n2 = static_cast<size_t>(V.op.n_rho) * V.op.n_T;
+        const size_t prof = static_cast<size_t>(NR) * NZ;
+        const size_t bytes = sizeof(ColumnIn) * NR + sizeof(double) * (3 * n2 + NR + 2 * prof);
+        char* d = nullptr;
+        hipError_t e = hipMalloc(reinterpret_cast<void**>(&d), bytes);
+        if (e != hipSuccess) throw std::runtime_error(std::string("vertical columns: ") + hipGetErrorString(e));
+        auto* din = reinterpret_cast<ColumnIn*>(d);
+        double* dt = reinterpret_cast<double*>(d + sizeof(ColumnIn) * NR);
+        double* dz = dt + 3 * n2;
+        double* drho = dz + NR;
+        double* dT = drho + prof;
+        ColumnTables t = th;
+        t.ross = dt;
+        t.es = dt + n2;
+        t.mu = dt + 2 * n2;
+        e = hipMemcpyAsync(din, in.data(), sizeof(ColumnIn) * NR, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(dt, th.ross, sizeof(double) * n2, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(dt + n2, th.es, sizeof(double) * n2, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(dt + 2 * n2, th.mu, sizeof(double) * n2, hipMemcpyHostToDevice, st);
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        if (e == hipSuccess) e = hipEventCreate(&ev[0]);
+        if (e == hipSuccess) e = hipEventCreate(&ev[1]);
+        if (e == hipSuccess) e = hipEventRecord(ev[0], st);
+        if (e == hipSuccess) e = launch_vol_columns(t, din, NR, NZ, V.prm.tau_mid, dz, drho, dT, st);
+        if (e == hipSuccess) e = hipEventRecord(ev[1], st);
+        if (e == hipSuccess) e = hipMemcpyAsync(V.z_max.data(), dz, sizeof(double) * NR, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(V.rho_profile.data(), drho, sizeof(double) * prof, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(V.T_profile.data(), dT, sizeof(double) * prof, hipMemcpyDeviceToHost, st);
+        const hipError_t es = hipStreamSynchronize(st);
+        if (e == hipSuccess) e = es;
+        float ms = 0.0f;
+        if (e == hipSuccess && hipEventElapsedTime(&ms, ev[0], ev[1]) == hipSuccess) V.build_ms_columns_kernel = ms;
+        for (auto& x : ev)
+            if (x) (void)hipEventDestroy(x);
+        const hipError_t ef = hipFree(d);
+        if (e == hipSuccess) e = ef;
+        if (e != hipSuccess) throw std::runtime_error(std::string("vertical columns on the GPU: ") + hipGetErrorString(e));
     }
 
     // src/volumetric_disk.cpp:783-849
@@ -370,7 +371,7 @@ struct Builder {
 }  // namespace
 
 void build_volumetric(double M, double a, double r_outer, double t_peak, const VolParams& p,
-                      VolTables& V, const hipStream_t* device_stream) {
+                      VolTables& V, const hipStream_t* device_stream, bool device_columns) {
     const auto t0 = std::chrono::steady_clock::now();
     V.M = M;
     V.a = a;
@@ -413,7 +414,9 @@ void build_volumetric(double M, double a, double r_outer, double t_peak, const V
     const auto t2 = std::chrono::steady_clock::now();
     Builder b{V};
     b.radial();
-    b.vertical();
+    const auto tc0 = std::chrono::steady_clock::now();
+    b.vertical(device_columns ? device_stream : nullptr);
+    V.build_ms_columns = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tc0).count();
     b.normalize();
     const auto t3 = std::chrono::steady_clock::now();
     V.build_ms_opacity = std::chrono::duration<double, std::milli>(t2 - t1).count();

@@ -57,12 +57,16 @@ struct VolTables {
     OpacityTables op;
     int perm[512];
     double build_ms_opacity = 0.0, build_ms_total = 0.0;   // wall time of the set-up
+    double build_ms_columns = 0.0;       // wall time of the vertical columns (host or GPU)
+    double build_ms_columns_kernel = 0.0;   // GPU build: the column kernel alone (events)
 };
 
 // Builds everything (src/volumetric_disk.cpp:33-73 construction order).
-// With a stream, the opacity tables are computed on the GPU on that stream;
-// without one, on host threads.  Either way the tables end in host memory.
+// With a stream, the opacity tables are computed on the GPU on that stream,
+// and the vertical columns too unless device_columns is false; without one,
+// both on host threads.  The radial structure and the normalisation are
+// host code.  Either way the tables end in host memory.
 void build_volumetric(double M, double a, double r_outer, double t_peak, const VolParams& p,
-                      VolTables& out, const hipStream_t* device_stream = nullptr);
+                      VolTables& out, const hipStream_t* device_stream = nullptr, bool device_columns = true);
 
 }  // namespace grrt_amd

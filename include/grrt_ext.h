@@ -172,13 +172,19 @@ GRRT_EXPORT int grrt_ext_render_spectral_f32(GRRTContext* ctx, float* out, int w
 typedef struct GRRTVolTables GRRTVolTables;
 GRRT_EXPORT GRRTVolTables* grrt_ext_vol_build(const GRRTParams* params);
 /* The same with the opacity tables (10^4 independent Saha solves and their
- * absorption and Rosseland opacities) computed on the GPU, as grrt_create
- * does; NULL without a GPU. */
+ * absorption and Rosseland opacities) and the 500 vertical columns computed
+ * on the GPU, as grrt_create does; NULL without a GPU. */
 GRRT_EXPORT GRRTVolTables* grrt_ext_vol_build_device(const GRRTParams* params);
+/* Either part on the GPU: flags bit 0, the opacity tables; bit 1, the
+ * vertical columns (which needs bit 0).  0: grrt_ext_vol_build; 3:
+ * grrt_ext_vol_build_device; 1: opacity tables on the GPU, columns on host
+ * threads, which equal the GPU's columns bit for bit. */
+GRRT_EXPORT GRRTVolTables* grrt_ext_vol_build_ex(const GRRTParams* params, int flags);
 /* Scalars in this order: r_isco, r_horizon, r_min, r_outer, taper_width,
  * E_isco, L_isco, rho_scale, noise_scale, turbulence, octaves, n_r, n_z,
  * n_nu, log10 nu_min, log10 nu_max, M, a, t_peak, then the set-up's wall
- * time in ms: opacity tables, whole build.  Returns the count. */
+ * time in ms: opacity tables, whole build, vertical columns, and the GPU
+ * column kernel alone (0 for a host build).  Returns the count. */
 GRRT_EXPORT int grrt_ext_vol_scalars(const GRRTVolTables* t, double* out, int cap);
 /* Copies table `name` ("H", "z_max", "rho_mid", "T_eff", "rho_profile",
  * "T_profile", "kappa_abs", "kappa_es", "kappa_ross", "mu", "perm") into

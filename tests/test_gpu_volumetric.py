@@ -34,9 +34,10 @@ def _setup(grrt, oracle, W, H, spin=0.9, theta=75.0, freqs=None, turbulence=0.4)
     p = grrt.make_params(width=W, height=H, metric=1, spin=spin, observer_theta=math.radians(theta),
                          fov=math.radians(90), volumetric=True, disk_outer=20.0, turbulence=turbulence,
                          seed=42, octaves=2, alpha=0.1, freqs=freqs)
-    # the tables grrt_create builds (opacity tables on the GPU, which agree
-    # with the host build to ~1e-12 and are deterministic), so the oracle and
-    # the tracer read identical disks
+    # the tables grrt_create builds (opacity tables and vertical columns on
+    # the GPU, deterministic; the opacities agree with the host build to
+    # ~1e-12 and the columns equal the host's over the same opacities), so
+    # the oracle and the tracer read identical disks
     T = grrt.volumetric_tables(p, device=True)
     V = oracle.vol_struct(T)
     sc = oracle.scene(width=W, height=H, spin_a=spin, theta_obs=math.radians(theta),
@@ -323,12 +324,13 @@ def test_volumetric_step_doubling_parity(grrt, oracle):
 
 
 def test_device_opacity_tables_equal_host(grrt):
-    """grrt_create computes the opacity tables on the GPU (one lane per
-    (rho, T) cell); the host builder runs the same point functions on host
-    threads.  The device exp/pow/log differ from the host's in the last bits,
-    so the tables agree to rounding, and so do the radial and vertical
-    structure built from them; the Saha passes' 2-cycles (partial
-    ionisation) are stable, so no cell drifts."""
+    """grrt_create computes the opacity tables (one lane per (rho, T) cell)
+    and the vertical columns (one wavefront per column) on the GPU; the host
+    builder runs the same point functions and column steps on host threads.
+    The device exp/pow/log of the opacity point functions differ from the
+    host's in the last bits, so the tables agree to rounding, and so do the
+    radial and vertical structure built from them; the Saha passes' 2-cycles
+    (partial ionisation) are stable, so no cell drifts."""
     print("case: opacity tables host vs GPU, spectral cube scene (60 frequencies) and default disk", flush=True)
     for p in (grrt.make_params(width=64, height=64, metric=1, spin=0.9, volumetric=True, disk_outer=20.0,
                                turbulence=0.4, seed=42, octaves=2, alpha=0.1, freqs=FREQS),
@@ -347,3 +349,27 @@ def test_device_opacity_tables_equal_host(grrt):
         np.testing.assert_allclose(dev["rho_profile"], host["rho_profile"], rtol=0, atol=1e-9)
         assert dev["rho_scale"] == pytest.approx(host["rho_scale"], rel=1e-9)
         assert np.array_equal(dev["perm"], host["perm"])
+
+
+def test_device_columns_equal_host_columns_bit_for_bit(grrt):
+    """Over the same (GPU-built) opacity tables, the 500 vertical columns
+    built on the GPU equal the host threads' columns bit for bit: z_max,
+    the density and the temperature profiles (both builds run the column
+    steps of vol_column.hpp with IEEE +, -, *, /, sqrt only and no fused
+    operations, so they round alike)."""
+    print("case: vertical columns GPU vs host over the GPU opacity tables", flush=True)
+    for p in (grrt.make_params(width=64, height=64, metric=1, spin=0.9, volumetric=True, disk_outer=20.0,
+                               turbulence=0.4, seed=42, octaves=2, alpha=0.1, freqs=FREQS),
+              grrt.make_params(metric=1, spin=0.998, volumetric=True, disk_outer=30.0),
+              grrt.make_params(metric=1, spin=0.0, volumetric=True, disk_outer=20.0, alpha=0.3)):
+        dev = grrt.volumetric_tables(p, device=True)
+        hcol = grrt.volumetric_tables(p, device=True, device_columns=False)
+        for name in ("kappa_es", "kappa_ross", "mu"):
+            assert np.array_equal(dev[name], hcol[name]), name      # the same GPU opacity tables
+        for name in ("z_max", "rho_profile", "T_profile", "H", "rho_mid", "T_eff"):
+            same = dev[name] == hcol[name]
+            print(f"{name}: {np.sum(~same)} of {same.size} entries differ", flush=True)
+            assert np.array_equal(dev[name], hcol[name]), name
+        assert dev["rho_scale"] == hcol["rho_scale"]
+        print(f"total set-up: GPU columns {dev['build_ms_total']:.2f} ms, host columns "
+              f"{hcol['build_ms_total']:.2f} ms", flush=True)
