@@ -307,8 +307,8 @@ def batched_views_section(grrt, reps: int) -> dict:
 
 
 CUBE_FREQS = [10 ** (14 + 4 * k / 31) for k in range(32)]
-# The cube's longest ray (pixel x, y): 4159 Dormand-Prince attempts and 3883
-# ray-march steps over 3877 entries into the disk volume (CPU oracle).
+# The cube's longest ray (pixel x, y): 4057 Dormand-Prince attempts and 3777
+# ray-march steps over 3771 entries into the disk volume (CPU oracle).
 LONGEST_CUBE_RAY = (274, 253)
 
 
