@@ -405,6 +405,29 @@ __device__ double critical_score(double a, double M, double P0, double P3, doubl
     return lo;
 }
 
+// Critical-curve flags, one lane per sample: the camera covector of
+// camera_ray (its two pixel angles' sin/cos by the branch-free Cephes form;
+// the gate needs ~1e-9, not the integrator's last bit), then the score.
+__global__ __launch_bounds__(256) void critical_flags_kernel(const TraceParams* __restrict__ Pg, long long n,
+                                                              double score_max, double r0, double r1, double sth,
+                                                              double cth, uint8_t* __restrict__ flags) {
+    const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const ViewParams& V = Pg->view;
+    SampleId d;
+    decode_view_ray(V, i, d);
+    double px, py, sa, ca_, sb, cb;
+    sample_position(V, d, px, py);
+    sincos_cephes((px - V.W / 2.0) * V.fov / V.W, sa, ca_);
+    sincos_cephes((py - V.H / 2.0) * V.fov / V.W, sb, cb);
+    const double ca = cb * sa, cc = cb * ca_;
+    const double p0 = -V.e0t + (-ca * V.e1[0] - sb * V.e2[0] + cc * V.e3[0]);
+    const double p2 = -ca * V.e1[2] - sb * V.e2[2] + cc * V.e3[2];
+    const double p3 = -ca * V.e1[3] - sb * V.e2[3] + cc * V.e3[3];
+    const double P0 = V.g00 * p0 + V.g03 * p3, P3 = V.g03 * p0 + V.g33 * p3;
+    flags[i] = fabs(critical_score(V.a, V.M, P0, P3, V.g22 * p2, sth, cth, r0, r1)) < score_max ? 1 : 0;
+}
+
 __device__ __forceinline__ f32x16 window_block(const bf16x8* W, const bf16x8 (&in)[16], const float* bias, int mt,
                                                int g, int lane) {
     f32x16 a = bias_tile(bias, mt, g, kHid);
@@ -624,37 +647,21 @@ __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
         // 113-130): shade with that g, and send the ray to the integrator when
         // the network's own ln g disagrees with it by more than lng_tol.
         // Rays near the critical curve (the shadow's edge) go to the
-        // integrator whatever the network predicts.
+        // integrator whatever the network predicts (flags written before
+        // the pass).
         double rc = 0.0, g_exact = 0.0;
-        bool inconsistent = false, near_critical = false;
-        if (valid && (cls == GRRT_CLASS_DISK || IA.crit_max > 0.0)) {
+        bool inconsistent = false;
+        const bool near_critical = valid && IA.crit_flags && IA.crit_flags[i] != 0;
+        if (valid && cls == GRRT_CLASS_DISK) {
             SampleId d;
             decode_view_ray(V, i, d);
-            if (cls == GRRT_CLASS_DISK) {
-                double P0, P3;
-                Vec5 y0;
-                camera_ray(V, d, P0, P3, y0);
-                rc = exp(static_cast<double>(y[3]));
-                g_exact = disk_redshift(V, rc, P0, P3);
-                inconsistent = IA.lng_tol > 0.0 &&
-                               !(fabs(log(fmax(g_exact, 1e-300)) - static_cast<double>(y[4])) <= IA.lng_tol);
-            }
-            if (IA.crit_max > 0.0) {
-                // the camera covector of camera_ray, its two pixel angles'
-                // sin/cos by the branch-free Cephes form (the gate needs
-                // ~1e-9, not the integrator's last bit)
-                double px, py, sa, ca_, sb, cb;
-                sample_position(V, d, px, py);
-                sincos_cephes((px - V.W / 2.0) * V.fov / V.W, sa, ca_);
-                sincos_cephes((py - V.H / 2.0) * V.fov / V.W, sb, cb);
-                const double ca = cb * sa, cc = cb * ca_;
-                const double p0 = -V.e0t + (-ca * V.e1[0] - sb * V.e2[0] + cc * V.e3[0]);
-                const double p2 = -ca * V.e1[2] - sb * V.e2[2] + cc * V.e3[2];
-                const double p3 = -ca * V.e1[3] - sb * V.e2[3] + cc * V.e3[3];
-                const double P0 = V.g00 * p0 + V.g03 * p3, P3 = V.g03 * p0 + V.g33 * p3;
-                near_critical = fabs(critical_score(V.a, V.M, P0, P3, V.g22 * p2, IA.crit_sth, IA.crit_cth,
-                                                    IA.crit_r0, IA.crit_r1)) < IA.crit_max;
-            }
+            double P0, P3;
+            Vec5 y0;
+            camera_ray(V, d, P0, P3, y0);
+            rc = exp(static_cast<double>(y[3]));
+            g_exact = disk_redshift(V, rc, P0, P3);
+            inconsistent = IA.lng_tol > 0.0 &&
+                           !(fabs(log(fmax(g_exact, 1e-300)) - static_cast<double>(y[4])) <= IA.lng_tol);
         }
         const bool fb = valid && (static_cast<double>(H) > IA.entropy_max || inconsistent || near_critical);
         const long long pf = wave_append(fb, &IA.counts[0]);
@@ -1018,6 +1025,14 @@ hipError_t launch_mlp_forward(const MlpArgs& a, bool train, hipStream_t st) {
     const dim3 grid(blocks_for(a.n, 4 * 32));
     if (train) hipLaunchKernelGGL((mlp_forward_kernel<1, true>), grid, dim3(256), 0, st, a);
     else hipLaunchKernelGGL((mlp_forward_kernel<1, false>), grid, dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_critical_flags(const TraceParams* P, long long n, double score_max, double r0, double r1,
+                                 double sth, double cth, uint8_t* flags, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(critical_flags_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, P, n, score_max, r0, r1,
+                       sth, cth, flags);
     return hipGetLastError();
 }
 

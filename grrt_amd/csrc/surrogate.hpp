@@ -103,11 +103,9 @@ struct InferArgs {
     const TraceParams* P;               // the frame's trace parameters (device memory)
     double entropy_max;                 // nats; above it the ray falls back
     double lng_tol;                     // |ln g_net - ln g(r_net)| above it falls back (<= 0: off)
-    // Critical-curve gate: a ray whose |critical_score| is below crit_max
-    // falls back (<= 0: off); crit_r0, crit_r1: the view's prograde and
-    // retrograde equatorial photon-orbit radii, the range of the spherical
-    // photon orbits; crit_sth, crit_cth: sin and cos of the observer's theta.
-    double crit_max, crit_r0, crit_r1, crit_sth, crit_cth;
+    // Critical-curve gate: sample i falls back when crit_flags[i] != 0
+    // (written by launch_critical_flags; nullptr: no gate).
+    const uint8_t* crit_flags;
     uint32_t judge_threshold;           // hash < threshold (of 2^32) selects a judged ray
     uint32_t frame_seed;
     long long ray_base;                 // first ray of the launch (rays ray_base .. ray_base + mlp.n)
@@ -135,6 +133,13 @@ hipError_t launch_pack(const float* params, __bf16* pack0, __bf16* packH, __bf16
                        hipStream_t st);
 hipError_t launch_mlp_forward(const MlpArgs& a, bool train, hipStream_t st);
 hipError_t launch_mlp_infer(const InferArgs& a, int num_cus, hipStream_t st);
+// Critical-curve flags of the frame's n samples (P: the frame's trace
+// parameters in device memory): flags[i] = 1 when |critical score| of
+// sample i's camera ray is below score_max.  r0, r1: the view's prograde
+// and retrograde equatorial photon-orbit radii; sth, cth: sin and cos of
+// the observer's theta.
+hipError_t launch_critical_flags(const TraceParams* P, long long n, double score_max, double r0, double r1,
+                                 double sth, double cth, uint8_t* flags, hipStream_t st);
 hipError_t launch_loss_grad(const float* Y, const Target* T, const MlpArgs& rows, LossWeights w, float* dY,
                             double* loss_acc, hipStream_t st);
 hipError_t launch_make_dz(const float* dHT, const float* dY, const float* params, const float* ZT, long long B,

@@ -32,6 +32,7 @@ struct SurrogateState {
     DevBuf<long long> fb_list, judge_list;
     DevBuf<unsigned long long> counts;
     DevBuf<float> judge_err;
+    DevBuf<uint8_t> crit_flags;
     uint32_t frame_no = 0;
     unsigned long long last_fb = 0, last_judged = 0;   // list lengths of the last surrogate frame
     // Critical-curve gate of the surrogate frame: rays with |critical_score|
@@ -48,6 +49,7 @@ struct SurrogateState {
         fb_list.release();
         judge_list.release();
         counts.release();
+        crit_flags.release();
     }
 };
 
@@ -630,6 +632,16 @@ int grrt_ext_render_surrogate(GRRTContext* ctx, float* out, int device_out, doub
         for (int k = 1; k < nb; ++k) check(launch_reset(ctx->counters.ptr + k, nullptr, 0, st), "reset");
         check(launch_reset_counts(S.counts.ptr, 4 * nb, st), "reset counts");
         check(hipEventRecord(ctx->ev[0], st), "event");
+        if (S.crit_max > 0.0) {
+            // prograde and retrograde equatorial photon orbits (Bardeen, Press & Teukolsky 1972)
+            const double M = P.view.M, x = std::clamp(P.view.a / M, -1.0, 1.0);
+            const double rp = 2.0 * M * (1.0 + std::cos(2.0 / 3.0 * std::acos(-x)));
+            const double rm = 2.0 * M * (1.0 + std::cos(2.0 / 3.0 * std::acos(x)));
+            check(S.crit_flags.ensure(static_cast<size_t>(n_rays)), "hipMalloc(critical flags)");
+            check(launch_critical_flags(ctx->params_dev.ptr, n_rays, S.crit_max, std::min(rp, rm), std::max(rp, rm),
+                                        std::sin(P.view.th_obs), std::cos(P.view.th_obs), S.crit_flags.ptr, st),
+                  "critical flags");
+        }
         // The fallback launch takes the plain frame's launch shape (three
         // blocks per CU by default).  On the 2048^2 x 4 frame (~0.77 M listed
         // rays) that traces them in 5.7 ms against 6.5 ms with one block per
@@ -643,17 +655,7 @@ int grrt_ext_render_surrogate(GRRTContext* ctx, float* out, int device_out, doub
             IA.P = ctx->params_dev.ptr + k;
             IA.entropy_max = entropy_max;
             IA.lng_tol = lng_tol;
-            IA.crit_max = S.crit_max;
-            {
-                // prograde and retrograde equatorial photon orbits (Bardeen, Press & Teukolsky 1972)
-                const double M = P.view.M, x = std::clamp(P.view.a / M, -1.0, 1.0);
-                const double rp = 2.0 * M * (1.0 + std::cos(2.0 / 3.0 * std::acos(-x)));
-                const double rm = 2.0 * M * (1.0 + std::cos(2.0 / 3.0 * std::acos(x)));
-                IA.crit_r0 = std::min(rp, rm);
-                IA.crit_r1 = std::max(rp, rm);
-                IA.crit_sth = std::sin(P.view.th_obs);
-                IA.crit_cth = std::cos(P.view.th_obs);
-            }
+            IA.crit_flags = S.crit_max > 0.0 ? S.crit_flags.ptr : nullptr;
             IA.judge_threshold = thr;
             IA.frame_seed = frame;
             IA.ray_base = bands[k].r0;
