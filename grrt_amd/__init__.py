@@ -171,6 +171,8 @@ API = [
     ("grrt_ext_write_fits_cube", C.c_int, [C.c_char_p, _P, C.c_int, C.c_int, C.c_int, C.c_int,
                                            C.POINTER(C.c_double), C.POINTER(GRRTParams)]),
     ("grrt_ext_write_hdr", C.c_int, [C.c_char_p, C.POINTER(C.c_float), C.c_int, C.c_int, C.c_int]),
+    ("grrt_ext_read_texture", C.c_int, [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_float),
+                                        C.c_int64]),
     ("grrt_ext_render_hdr", C.c_int, [_P, C.c_char_p, C.c_char_p, C.POINTER(C.c_float), PROGRESS_FN, C.c_void_p]),
     ("grrt_ext_surrogate_init", C.c_int, [_P, C.c_uint64]),
     ("grrt_ext_surrogate_generate", C.c_int, [_P, C.c_int64, C.c_uint64, C.c_int]),
@@ -244,7 +246,7 @@ def make_params(width=256, height=256, metric=METRIC_KERR, spin=0.998, mass=1.0,
                 fov=1.5707963, disk=True, disk_inner=0.0, disk_outer=20.0,
                 disk_temperature=0.0, stars=False, tolerance=1e-8, max_steps=10000,
                 spp=1, volumetric=False, alpha=0.1, turbulence=0.4, seed=42,
-                noise_scale=0.0, octaves=2, freqs=None) -> GRRTParams:
+                noise_scale=0.0, octaves=2, freqs=None, texture=None) -> GRRTParams:
     p = GRRTParams()
     p.width, p.height = width, height
     p.metric_type = metric
@@ -254,6 +256,10 @@ def make_params(width=256, height=256, metric=METRIC_KERR, spin=0.998, mass=1.0,
     p.disk_inner, p.disk_outer, p.disk_temperature = disk_inner, disk_outer, disk_temperature
     p.disk_volumetric = 1 if volumetric else 0
     p.background_type = BG_STARS if stars else BG_BLACK
+    if texture is not None:
+        p.background_type = BG_TEXTURE
+        p._texture_keepalive = texture.encode()
+        p.background_texture_path = p._texture_keepalive
     p.integrator_tolerance, p.integrator_max_steps = tolerance, max_steps
     p.samples_per_pixel = spp
     p.backend = BACKEND_GPU
@@ -676,6 +682,18 @@ def write_fits_cube(path: str, cube: np.ndarray, freqs=None, params: GRRTParams 
                                         C.byref(params) if params is not None else None)
     if rc != 0:
         raise RuntimeError(lib().grrt_last_error().decode())
+
+
+def read_texture(path: str) -> np.ndarray:
+    """Read a GRRT_BG_TEXTURE image (.hdr or PFM) as (height, width, 3) float32 linear RGB."""
+    w, h = C.c_int(0), C.c_int(0)
+    if lib().grrt_ext_read_texture(path.encode(), C.byref(w), C.byref(h), None, 0) != 0:
+        raise RuntimeError(lib().grrt_last_error().decode())
+    out = np.empty((h.value, w.value, 3), np.float32)
+    if lib().grrt_ext_read_texture(path.encode(), C.byref(w), C.byref(h),
+                                   out.ctypes.data_as(C.POINTER(C.c_float)), out.size) != 0:
+        raise RuntimeError(lib().grrt_last_error().decode())
+    return out
 
 
 def write_hdr(path: str, rgba: np.ndarray, normalised: bool = False) -> None:

@@ -144,6 +144,29 @@ void build_scene(GRRTContext* ctx) {
         upload(ctx->star_start, t.start, ctx->stream);
         upload(ctx->star_tpb, t.tpb, ctx->stream);
     }
+    const char* tex_path = ctx->params.background_texture_path;
+    ctx->bg_on = false;
+    if (r.texture && tex_path && *tex_path) {
+        // The reference declares this mode (GRRT_BG_TEXTURE with
+        // background_texture_path, include/grrt/types.h:15-17) and renders
+        // it black (src/api.cpp:136 builds the sphere for GRRT_BG_STARS
+        // only); with no path that stays so.  With a path the image is read
+        // once per path and kept in device memory.
+        const char* path = tex_path;
+        if (ctx->bg_tex.ptr == nullptr || ctx->bg_path != path) {
+            Texture tex;
+            try {
+                tex = read_texture(path);
+            } catch (const std::exception& e) {
+                throw HipFailure(e.what());
+            }
+            upload(ctx->bg_tex, tex.rgb, ctx->stream);
+            ctx->bg_w = tex.width;
+            ctx->bg_h = tex.height;
+            ctx->bg_path = path;
+        }
+        ctx->bg_on = true;
+    }
     if (r.volumetric) build_volumetric_tables(ctx);
     ctx->sampling_spp = r.spp;
     check(hipStreamSynchronize(ctx->stream), "upload tables");
@@ -218,6 +241,13 @@ float live_progress(const GRRTContext* ctx) {
     while (p > seen && !ctx->progress.compare_exchange_weak(seen, p)) {
     }
     return std::max(p, seen);
+}
+
+void set_background(const GRRTContext* ctx, grrt_amd::TraceParams& P) {
+    const bool on = ctx->bg_on && ctx->bg_tex.ptr;
+    P.bg_tex = on ? ctx->bg_tex.ptr : nullptr;
+    P.bg_w = on ? ctx->bg_w : 0;
+    P.bg_h = on ? ctx->bg_h : 0;
 }
 
 void begin_call(GRRTContext* ctx) {
@@ -320,6 +350,7 @@ int render_window(GRRTContext* ctx, int x0, int y0, int tw, int th, void* d_out,
     P.stars = r.stars;
     P.star_start = ctx->star_start.ptr;
     P.star_tpb = ctx->star_tpb.ptr;
+    set_background(ctx, P);
     P.sample_rgb = fs ? fs->rgb : ctx->sample_rgb.ptr;
     P.sample_cls = fs ? fs->cls : ctx->sample_cls.ptr;
     P.records = ctx->keep_records ? ctx->records.ptr : nullptr;
@@ -1280,6 +1311,7 @@ void prepare_batch(GRRTContext* ctx, const GRRTView* views, int n, uint64_t seed
     P.stars = r.stars;
     P.star_start = ctx->star_start.ptr;
     P.star_tpb = ctx->star_tpb.ptr;
+    set_background(ctx, P);
     P.sample_rgb = ctx->batch_rgb.ptr;
     P.sample_cls = ctx->batch_cls.ptr;
     if (ctx->keep_records) check(ctx->batch_records.ensure(static_cast<size_t>(rays)), "hipMalloc(batch records)");
@@ -1446,6 +1478,23 @@ int grrt_ext_write_fits_cube(const char* path, const void* cube, int bitpix, int
 int grrt_ext_render_hdr(GRRTContext* ctx, const char* path_linear, const char* path_scaled, float* framebuffer,
                         grrt_progress_fn progress, void* user_data) {
     return guarded(ctx, [&] { return render_hdr(ctx, path_linear, path_scaled, framebuffer, progress, user_data); });
+}
+
+int grrt_ext_read_texture(const char* path, int* width, int* height, float* rgb, int64_t capacity) {
+    try {
+        if (!path || !width || !height) throw std::runtime_error("null argument");
+        const grrt_amd::Texture t = grrt_amd::read_texture(path);
+        *width = t.width;
+        *height = t.height;
+        if (rgb) {
+            if (capacity < static_cast<int64_t>(t.rgb.size())) throw std::runtime_error("texture buffer too small");
+            std::memcpy(rgb, t.rgb.data(), t.rgb.size() * sizeof(float));
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        g_last_error = e.what();
+        return -1;
+    }
 }
 
 int grrt_ext_write_hdr(const char* path, const float* rgba, int width, int height, int normalised) {

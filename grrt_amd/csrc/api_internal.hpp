@@ -60,6 +60,10 @@ float live_progress(const GRRTContext* ctx);
 // Start of a rendering API call: clears the cancel request and the progress.
 void begin_call(GRRTContext* ctx);
 
+// The texture background of the context's scene into a launch block (or
+// none).
+void set_background(const GRRTContext* ctx, grrt_amd::TraceParams& P);
+
 // Copies a device frame of n floats to the caller's host buffer through the
 // context's pinned staging buffer (one DMA copy, then a threaded host copy).
 void frame_to_host(GRRTContext* ctx, float* host, const float* dev, size_t n);

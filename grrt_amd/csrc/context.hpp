@@ -83,6 +83,10 @@ struct GRRTContext {
     double flux_max = 0.0;
     grrt_amd::DevBuf<double> flux, spec, star_tpb;
     grrt_amd::DevBuf<int> star_start;
+    grrt_amd::DevBuf<float> bg_tex;          // equirectangular background, RGB, row 0 at theta = 0
+    int bg_w = 0, bg_h = 0;
+    std::string bg_path;                     // the file bg_tex was read from
+    bool bg_on = false;                      // the scene samples bg_tex for escaped rays
 
     // Volumetric disk tables in device memory.
     grrt_amd::VolDev vol{};

@@ -30,6 +30,7 @@ Resolved resolve_params(const GRRTParams& p) {
     r.r_out = p.disk_outer > 0.0 ? p.disk_outer : 20.0;
     r.r_in = p.disk_inner > 0.0 ? p.disk_inner : r.r_isco;
     r.stars = p.background_type == GRRT_BG_STARS ? 1 : 0;
+    r.texture = p.background_type == GRRT_BG_TEXTURE ? 1 : 0;
     r.spp = p.samples_per_pixel > 0 ? p.samples_per_pixel : 1;
     return r;
 }

@@ -21,6 +21,7 @@ struct Resolved {
     int volumetric;
     double r_in, r_out, t_disk;
     int stars;
+    int texture;              // equirectangular background image (GRRT_BG_TEXTURE)
     int spp;
     double r_h, r_isco;
 };

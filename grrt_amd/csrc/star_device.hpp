@@ -44,4 +44,44 @@ __device__ __noinline__ double star_sample(const TraceParams& P, double theta, d
     return 0.0;
 }
 
+// The texture background at an escape direction (theta, phi), folded as
+// star_sample folds it (theta into [0, pi], phi into [-pi, pi]): bilinear
+// between texel centres, wrapping in phi and clamped at the poles.  Adds
+// the texel's linear RGB to col.
+__device__ __noinline__ void texture_sample(const TraceParams& P, double theta, double phi, double col[3]) {
+    const double twopi = 2.0 * kPi;
+    theta = fmod(theta, twopi);
+    if (theta < 0.0) theta += twopi;
+    if (theta > kPi) {
+        theta = twopi - theta;
+        phi += kPi;
+    }
+    phi = fmod(phi, twopi);
+    if (phi > kPi) phi -= twopi;
+    if (phi < -kPi) phi += twopi;
+    const int W = P.bg_w, H = P.bg_h;
+    const double x = (phi + kPi) / twopi * W - 0.5, y = theta / kPi * H - 0.5;
+    const double fx0 = floor(x), fy0 = floor(y);
+    const double tx = x - fx0;
+    double ty = y - fy0;
+    int x0 = static_cast<int>(fx0) % W;
+    if (x0 < 0) x0 += W;
+    const int x1 = x0 + 1 < W ? x0 + 1 : 0;
+    int y0 = static_cast<int>(fy0), y1 = y0 + 1;
+    if (y0 < 0) {
+        y0 = y1 = 0;
+        ty = 0.0;
+    } else if (y1 > H - 1) {
+        y0 = y1 = H - 1;
+        ty = 0.0;
+    }
+    const float* r0 = P.bg_tex + static_cast<long long>(y0) * W * 3;
+    const float* r1 = P.bg_tex + static_cast<long long>(y1) * W * 3;
+    for (int k = 0; k < 3; ++k) {
+        const double a = (1.0 - tx) * r0[3 * x0 + k] + tx * r0[3 * x1 + k];
+        const double b = (1.0 - tx) * r1[3 * x0 + k] + tx * r1[3 * x1 + k];
+        col[k] += (1.0 - ty) * a + ty * b;
+    }
+}
+
 }  // namespace grrt_amd

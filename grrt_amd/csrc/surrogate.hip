@@ -675,13 +675,17 @@ __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
             double col[3] = {0.0, 0.0, 0.0};
             if (cls == GRRT_CLASS_DISK) {
                 thin_disk_color(V, s_flux, P.spec_lut, rc, g_exact, col);
-            } else if (cls == GRRT_CLASS_ESCAPED && P.stars) {
+            } else if (cls == GRRT_CLASS_ESCAPED && (P.stars || P.bg_tex)) {
                 const double th = atan2(static_cast<double>(y[5]), static_cast<double>(y[6]));
                 const double ph = atan2(static_cast<double>(y[7]), static_cast<double>(y[8])) + V.ph_obs;
-                const double bb = star_sample(P, th, ph);
-                col[0] += bb;
-                col[1] += bb;
-                col[2] += bb;
+                if (P.stars) {
+                    const double bb = star_sample(P, th, ph);
+                    col[0] += bb;
+                    col[1] += bb;
+                    col[2] += bb;
+                } else {
+                    texture_sample(P, th, ph, col);
+                }
             }
             P.sample_rgb[3 * i + 0] = col[0];
             P.sample_rgb[3 * i + 1] = col[1];

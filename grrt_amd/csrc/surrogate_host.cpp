@@ -179,6 +179,7 @@ TraceParams frame_params(GRRTContext* ctx, long long n_rays) {
     P.stars = r.stars;
     P.star_start = ctx->star_start.ptr;
     P.star_tpb = ctx->star_tpb.ptr;
+    grrt_api::set_background(ctx, P);
     P.sample_rgb = ctx->sample_rgb.ptr;
     P.sample_cls = ctx->sample_cls.ptr;
     P.records = nullptr;

@@ -120,6 +120,10 @@ struct TraceParams {
     int stars;
     const int* star_start;              // kStarCells + 1
     const double* star_tpb;             // (theta, phi, brightness) in cell order
+    // Texture background (GRRT_BG_TEXTURE): equirectangular linear RGB,
+    // bg_w x bg_h, row 0 at theta = 0, column 0 at phi = -pi; nullptr: none.
+    const float* bg_tex;
+    int bg_w, bg_h;
 
     // Outputs.
     double* sample_rgb;                 // n_rays * 3

@@ -125,6 +125,8 @@ __device__ __forceinline__ void finish_ray(const TraceParams& P, long long oid, 
         R.col[0] += b;
         R.col[1] += b;
         R.col[2] += b;
+    } else if (term == 1 && P.bg_tex) {
+        texture_sample(P, R.y.v[1], R.y.v[2], R.col);
     }
     const int cls = (R.n_cross > 0) ? GRRT_CLASS_DISK
                                     : (term == 0 ? GRRT_CLASS_CAPTURED

@@ -700,6 +700,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
             if (c >= 0) {
                 // stars: the thin tracer's catalogue search (src/renderer.cpp:56-61)
                 if (term == 1 && P.stars) mine += star_sample(P, s.y.v[1], s.y.v[2]);
+                if (term == 1 && P.bg_tex) {      // texture background: this lane's channel
+                    double t[3] = {0.0, 0.0, 0.0};
+                    texture_sample(P, s.y.v[1], s.y.v[2], t);
+                    mine += t[c];
+                }
                 P.sample_rgb[3 * id + c] = mine;
             }
             for (int k = 0; k < 3; ++k) col[k] = b.share(mine, k);

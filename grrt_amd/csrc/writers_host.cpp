@@ -307,3 +307,137 @@ float hdr_exposure_scale(const float* rgba, int width, int height) {
 }
 
 }  // namespace grrt_amd
+
+namespace grrt_amd {
+
+namespace {
+
+[[noreturn]] void bad_texture(const std::string& path, const std::string& why) {
+    throw std::runtime_error("background texture " + path + ": " + why);
+}
+
+// One header line (without the newline); false at end of file.
+bool read_line(std::FILE* f, std::string& line) {
+    line.clear();
+    int c;
+    while ((c = std::fgetc(f)) != EOF) {
+        if (c == '\n') return true;
+        line.push_back(static_cast<char>(c));
+        if (line.size() > 4096) return false;
+    }
+    return !line.empty();
+}
+
+Texture read_rgbe(std::FILE* f, const std::string& path) {
+    std::string line;
+    for (;;) {   // header lines up to the blank line
+        if (!read_line(f, line)) bad_texture(path, "truncated Radiance header");
+        if (line.empty()) break;
+        if (line.rfind("FORMAT=", 0) == 0 && line != "FORMAT=32-bit_rle_rgbe")
+            bad_texture(path, "unsupported " + line);
+    }
+    if (!read_line(f, line)) bad_texture(path, "missing resolution line");
+    int W = 0, H = 0;
+    char ys = 0, xs = 0;
+    if (std::sscanf(line.c_str(), "%cY %d %cX %d", &ys, &H, &xs, &W) != 4 || ys != '-' || xs != '+' || W <= 0 ||
+        H <= 0 || W > (1 << 16) || H > (1 << 16))
+        bad_texture(path, "resolution line must be -Y height +X width: " + line);
+    Texture t;
+    t.width = W;
+    t.height = H;
+    t.rgb.assign(static_cast<size_t>(W) * H * 3, 0.0f);
+    std::vector<unsigned char> px(static_cast<size_t>(W) * 4);
+    for (int y = 0; y < H; ++y) {
+        unsigned char head[4];
+        if (std::fread(head, 1, 4, f) != 4) bad_texture(path, "truncated pixel data");
+        const bool rle = W >= 8 && W < 32768 && head[0] == 2 && head[1] == 2 && ((head[2] << 8) | head[3]) == W &&
+                         !(head[2] & 0x80);
+        if (rle) {
+            for (int ch = 0; ch < 4; ++ch) {
+                int x = 0;
+                while (x < W) {
+                    int count = std::fgetc(f);
+                    if (count == EOF) bad_texture(path, "truncated run-length scanline");
+                    if (count > 128) {
+                        count -= 128;
+                        const int v = std::fgetc(f);
+                        if (v == EOF || x + count > W) bad_texture(path, "bad run in scanline");
+                        for (int k = 0; k < count; ++k) px[static_cast<size_t>(x++) * 4 + ch] = static_cast<unsigned char>(v);
+                    } else {
+                        if (count == 0 || x + count > W) bad_texture(path, "bad literal in scanline");
+                        for (int k = 0; k < count; ++k) {
+                            const int v = std::fgetc(f);
+                            if (v == EOF) bad_texture(path, "truncated scanline");
+                            px[static_cast<size_t>(x++) * 4 + ch] = static_cast<unsigned char>(v);
+                        }
+                    }
+                }
+            }
+        } else {   // flat scanline: the four bytes read are its first pixel
+            std::copy(head, head + 4, px.begin());
+            if (W > 1 && std::fread(px.data() + 4, 1, static_cast<size_t>(W - 1) * 4, f) != static_cast<size_t>(W - 1) * 4)
+                bad_texture(path, "truncated flat scanline");
+        }
+        float* row = t.rgb.data() + static_cast<size_t>(y) * W * 3;
+        for (int x = 0; x < W; ++x) {
+            const unsigned char* p = &px[static_cast<size_t>(x) * 4];
+            const float m = p[3] ? std::ldexp(1.0f, static_cast<int>(p[3]) - 136) : 0.0f;
+            for (int k = 0; k < 3; ++k) row[3 * x + k] = p[3] ? (p[k] + 0.5f) * m : 0.0f;   // Radiance's colr_color
+        }
+    }
+    return t;
+}
+
+Texture read_pfm(std::FILE* f, const std::string& path, bool colour) {
+    std::string line;
+    int W = 0, H = 0;
+    double scale = 0.0;
+    if (!read_line(f, line) || std::sscanf(line.c_str(), "%d %d", &W, &H) != 2 || W <= 0 || H <= 0 ||
+        W > (1 << 16) || H > (1 << 16))
+        bad_texture(path, "bad PFM size line");
+    if (!read_line(f, line) || std::sscanf(line.c_str(), "%lf", &scale) != 1 || scale == 0.0)
+        bad_texture(path, "bad PFM scale line");
+    const bool little = scale < 0.0;
+    const int nc = colour ? 3 : 1;
+    std::vector<unsigned char> raw(static_cast<size_t>(W) * H * nc * 4);
+    if (std::fread(raw.data(), 1, raw.size(), f) != raw.size()) bad_texture(path, "truncated PFM data");
+    Texture t;
+    t.width = W;
+    t.height = H;
+    t.rgb.assign(static_cast<size_t>(W) * H * 3, 0.0f);
+    for (int y = 0; y < H; ++y)          // stored bottom row first
+        for (int x = 0; x < W; ++x)
+            for (int k = 0; k < 3; ++k) {
+                const unsigned char* b = &raw[((static_cast<size_t>(H - 1 - y) * W + x) * nc + (colour ? k : 0)) * 4];
+                const uint32_t u = little ? (uint32_t(b[0]) | uint32_t(b[1]) << 8 | uint32_t(b[2]) << 16 | uint32_t(b[3]) << 24)
+                                          : (uint32_t(b[3]) | uint32_t(b[2]) << 8 | uint32_t(b[1]) << 16 | uint32_t(b[0]) << 24);
+                float v;
+                std::memcpy(&v, &u, 4);
+                t.rgb[(static_cast<size_t>(y) * W + x) * 3 + k] = v;
+            }
+    return t;
+}
+
+}  // namespace
+
+Texture read_texture(const std::string& path) {
+    std::FILE* f = std::fopen(path.c_str(), "rb");
+    if (!f) bad_texture(path, "cannot open");
+    Texture t;
+    try {
+        std::string first;
+        if (!read_line(f, first)) bad_texture(path, "empty file");
+        if (first == "#?RADIANCE" || first == "#?RGBE") t = read_rgbe(f, path);
+        else if (first == "PF" || first == "Pf") t = read_pfm(f, path, first == "PF");
+        else bad_texture(path, "not a Radiance .hdr or PFM image");
+    } catch (...) {
+        std::fclose(f);
+        throw;
+    }
+    std::fclose(f);
+    for (float& v : t.rgb)
+        if (!(v >= 0.0f) || !std::isfinite(v)) v = 0.0f;   // a background radiance is finite and non-negative
+    return t;
+}
+
+}  // namespace grrt_amd

@@ -86,4 +86,18 @@ struct ExposureSum {
 
 float hdr_exposure_scale(const float* rgba, int width, int height);
 
+// An equirectangular background image for GRRT_BG_TEXTURE: linear RGB
+// floats, row 0 at the top (theta = 0), column 0 at phi = -pi.
+struct Texture {
+    int width = 0, height = 0;
+    std::vector<float> rgb;   // width * height * 3
+};
+
+// Reads a Radiance RGBE .hdr ("-Y height +X width"; flat or new-style
+// run-length scanlines; value = (mantissa + 0.5) * 2^(exponent - 136), zero for exponent 0) or a PFM
+// (colour "PF" or grey "Pf", byte order from the scale's sign, rows stored
+// bottom first).  Throws std::runtime_error with the reason on anything
+// else.
+Texture read_texture(const std::string& path);
+
 }  // namespace grrt_amd

@@ -320,6 +320,13 @@ GRRT_EXPORT int grrt_ext_write_fits_cube(const char* path, const void* cube, int
  * first), linear (normalised=0) or scaled by 0.18 over the log-average
  * luminance of the lit pixels (normalised=1), the CLI's two HDR outputs. */
 GRRT_EXPORT int grrt_ext_write_hdr(const char* path, const float* rgba, int width, int height, int normalised);
+/* Host-only: reads the GRRT_BG_TEXTURE image format (an equirectangular
+ * Radiance RGBE .hdr or PFM, row 0 at theta = 0, column 0 at phi = -pi) into
+ * linear RGB floats, the form the tracer samples.  Writes the size; with rgb
+ * non-null also the width*height*3 values (capacity in floats).  The
+ * reference declares GRRT_BG_TEXTURE (include/grrt/types.h:17) without a
+ * reader.  Returns -1 with grrt_last_error() set on an unreadable file. */
+GRRT_EXPORT int grrt_ext_read_texture(const char* path, int* width, int* height, float* rgb, int64_t capacity);
 /* Renders the RGB frame with the rows streamed to the CLI's two HDR outputs
  * (replaces the render-then-write sequence of reference cli/main.cpp:386-434):
  * bands of rows are rendered on the GPU, and each finished band is encoded

@@ -53,3 +53,35 @@ def critical_score(a, M, p_t, p_th, p_ph, theta, n=None):
         u = r * r + a * a - a * b[:, None]
         out[sl] = ((u * u - (r * r - 2 * M * r + a * a) * k[:, None]) / r ** 4).min(axis=1)
     return out
+
+
+def write_pfm(path, img, little=True):
+    """Portable float map: "PF" (RGB) or "Pf" (grey) for an (h, w, 3) or
+    (h, w) array, the scale's sign giving the byte order (negative: little
+    endian), rows stored bottom first."""
+    img = np.asarray(img, np.float32)
+    h, w = img.shape[:2]
+    magic = b"PF\n" if img.ndim == 3 else b"Pf\n"
+    data = np.ascontiguousarray(img[::-1]).astype("<f4" if little else ">f4")
+    with open(path, "wb") as f:
+        f.write(magic + f"{w} {h}\n".encode() + (b"-1.0\n" if little else b"1.0\n") + data.tobytes())
+
+
+def direction_rgb(theta, phi):
+    """A smooth colour of the escape direction n = (sin t cos p, sin t sin p,
+    cos t): RGB = (1 + (n_z, n_y, n_x)) / 2.  Unchanged by the coordinate
+    fold theta -> 2 pi - theta, phi -> phi + pi, so it applies to raw final
+    coordinates; R fixes theta, G and B together fix phi."""
+    theta, phi = np.asarray(theta, np.float64), np.asarray(phi, np.float64)
+    st = np.sin(theta)
+    return 0.5 * (1.0 + np.stack([np.cos(theta), st * np.sin(phi), st * np.cos(phi)], -1))
+
+
+def direction_texture(width, height):
+    """direction_rgb at the texel centres of an equirectangular map: row j
+    at theta = (j + 1/2) pi / height (row 0 at the north pole), column i at
+    phi = -pi + (i + 1/2) 2 pi / width."""
+    th = (np.arange(height) + 0.5) * np.pi / height
+    ph = -np.pi + (np.arange(width) + 0.5) * 2.0 * np.pi / width
+    T, P = np.meshgrid(th, ph, indexing="ij")
+    return direction_rgb(T, P).astype(np.float32)

@@ -302,6 +302,31 @@ def test_critical_gate_sends_near_critical_rays_to_the_integrator(grrt, oracle):
     assert np.array_equal(scls[fbl], pcls[fbl]) and np.array_equal(srgb[fbl], prgb[fbl])
 
 
+def test_surrogate_texture_background(grrt, tmp_path):
+    """A constant texture: every escaped sample, shaded at the network's
+    predicted direction or traced by the integrator, carries exactly that
+    colour (in float); captured samples stay black."""
+    from parity_util import write_pfm
+    print("case: surrogate frame 96^2 a=0.998 with a constant texture", flush=True)
+    colour = np.float32([0.25, 0.5, 1.0])
+    path = str(tmp_path / "const.pfm")
+    write_pfm(path, np.broadcast_to(colour, (4, 8, 3)))
+    kw = dict(width=96, height=96, observer_theta=math.radians(80.0), fov=math.radians(90.0))
+    with _ctx(grrt, texture=path, **kw) as ctx:
+        ctx.set_sampling(grrt.SAMPLING_CENTER)
+        ctx.surrogate_init(3)
+        ctx.surrogate_generate(1 << 17, seed=31)
+        ctx.surrogate_train(steps=200, batch=2048, lr_max=2e-3, lr_min=1e-4, warmup=20)
+        fb, st = ctx.render_surrogate(entropy_max=0.1, judge_fraction=0.0)
+        cls, rgb = ctx.classes(), ctx.samples()
+    esc = cls == grrt.CLASS_ESCAPED
+    print(f"escaped {esc.sum()} of {cls.size}, fallback {st['fallback_rays']}", flush=True)
+    assert esc.sum() > 1000
+    assert np.array_equal(rgb[esc].astype(np.float32), np.broadcast_to(colour, (esc.sum(), 3)))
+    assert np.all(rgb[cls == grrt.CLASS_CAPTURED] == 0.0)
+    assert np.isfinite(fb).all()
+
+
 def _disk_pixel_flux_err(s_fb, p_fb, s_cls, p_cls, nS):
     disk = (s_cls.reshape(-1, nS) == 2).any(1) | (p_cls.reshape(-1, nS) == 2).any(1)
     a = s_fb.reshape(-1, 4)[disk, :3].sum(1).astype(np.float64)

@@ -171,3 +171,67 @@ def test_rgbe_round_trip(g, tmp_path, w):
     ref = fb[..., :3].astype(np.float64) * scale
     err2 = np.abs(rgb2 - ref) / np.where(m * scale > 1e-30, m * scale, 1.0)
     assert err2.max() <= 2.0 / 256
+
+
+# --- the GRRT_BG_TEXTURE reader (host only) -------------------------------
+
+@pytest.mark.parametrize("w", [5, 300])
+def test_texture_reader_reads_the_rgbe_writer(g, tmp_path, w):
+    """Flat (width < 8) and run-length scanlines: the reader returns what the
+    independent decoder above returns, value for value, row 0 first."""
+    rng = np.random.default_rng(4)
+    h = 6
+    fb = np.zeros((h, w, 4), np.float32)
+    fb[..., :3] = rng.lognormal(0, 3, (h, w, 3))
+    fb[2, :, :3] = 0.0
+    fb[..., 3] = 1.0
+    path = str(tmp_path / "t.hdr")
+    g.write_hdr(path, fb)
+    tex = g.read_texture(path)
+    ref = _read_rgbe(path).astype(np.float32)
+    assert tex.shape == (h, w, 3) and tex.dtype == np.float32
+    assert np.array_equal(tex, ref)
+    m = fb[..., :3].max(-1, keepdims=True)
+    assert (np.abs(tex - fb[..., :3]) / np.where(m > 0, m, 1.0)).max() <= 2.0 / 256
+    assert np.all(tex[2] == 0.0)
+
+
+@pytest.mark.parametrize("little", [True, False])
+def test_texture_reader_pfm(g, tmp_path, little):
+    from parity_util import write_pfm
+    rng = np.random.default_rng(5)
+    img = rng.uniform(0, 10, (7, 11, 3)).astype(np.float32)
+    path = str(tmp_path / "c.pfm")
+    write_pfm(path, img, little=little)
+    assert np.array_equal(g.read_texture(path), img)          # bottom-first rows restored
+    grey = rng.uniform(0, 1, (4, 9)).astype(np.float32)
+    write_pfm(path, grey, little=little)
+    assert np.array_equal(g.read_texture(path), np.repeat(grey[..., None], 3, -1))
+
+
+def test_texture_reader_clears_negative_and_non_finite(g, tmp_path):
+    from parity_util import write_pfm
+    img = np.full((2, 3, 3), 0.5, np.float32)
+    img[0, 0] = [-1.0, np.nan, np.inf]
+    path = str(tmp_path / "bad.pfm")
+    write_pfm(path, img)
+    tex = g.read_texture(path)
+    assert np.all(tex[0, 0] == 0.0) and np.all(tex.reshape(-1, 3)[1:] == 0.5)
+
+
+def test_texture_reader_errors(g, tmp_path):
+    cases = {
+        "missing.hdr": None,
+        "magic.hdr": b"P6\n2 2\n255\n" + bytes(12),
+        "orient.hdr": b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n+Y 2 +X 2\n" + bytes(16),
+        "short.hdr": b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n-Y 2 +X 2\n" + bytes(10),
+        "xyze.hdr": b"#?RADIANCE\nFORMAT=32-bit_rle_xyze\n\n-Y 1 +X 1\n" + bytes(4),
+        "short.pfm": b"PF\n2 2\n-1.0\n" + bytes(40),
+        "size.pfm": b"PF\n2 x\n-1.0\n" + bytes(48),
+    }
+    for name, raw in cases.items():
+        path = str(tmp_path / name)
+        if raw is not None:
+            open(path, "wb").write(raw)
+        with pytest.raises(RuntimeError, match=path.replace(".", r"\.")):
+            g.read_texture(path)
