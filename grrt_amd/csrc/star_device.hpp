@@ -44,11 +44,15 @@ __device__ __noinline__ double star_sample(const TraceParams& P, double theta, d
     return 0.0;
 }
 
+struct TexRGB {
+    double c[3];
+};
+
 // The texture background at an escape direction (theta, phi), folded as
 // star_sample folds it (theta into [0, pi], phi into [-pi, pi]): bilinear
-// between texel centres, wrapping in phi and clamped at the poles.  Adds
-// the texel's linear RGB to col.
-__device__ __noinline__ void texture_sample(const TraceParams& P, double theta, double phi, double col[3]) {
+// between texel centres, wrapping in phi and clamped at the poles; linear
+// RGB.  Returned by value so that callers keep their colour in registers.
+__device__ __noinline__ TexRGB texture_sample(const TraceParams& P, double theta, double phi) {
     const double twopi = 2.0 * kPi;
     theta = fmod(theta, twopi);
     if (theta < 0.0) theta += twopi;
@@ -77,11 +81,13 @@ __device__ __noinline__ void texture_sample(const TraceParams& P, double theta, 
     }
     const float* r0 = P.bg_tex + static_cast<long long>(y0) * W * 3;
     const float* r1 = P.bg_tex + static_cast<long long>(y1) * W * 3;
+    TexRGB out;
     for (int k = 0; k < 3; ++k) {
         const double a = (1.0 - tx) * r0[3 * x0 + k] + tx * r0[3 * x1 + k];
         const double b = (1.0 - tx) * r1[3 * x0 + k] + tx * r1[3 * x1 + k];
-        col[k] += (1.0 - ty) * a + ty * b;
+        out.c[k] = (1.0 - ty) * a + ty * b;
     }
+    return out;
 }
 
 }  // namespace grrt_amd

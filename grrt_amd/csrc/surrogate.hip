@@ -684,7 +684,10 @@ __global__ __launch_bounds__(256, 2) void mlp_infer_kernel(InferArgs IA) {
                     col[1] += bb;
                     col[2] += bb;
                 } else {
-                    texture_sample(P, th, ph, col);
+                    const TexRGB t = texture_sample(P, th, ph);
+                    col[0] += t.c[0];
+                    col[1] += t.c[1];
+                    col[2] += t.c[2];
                 }
             }
             P.sample_rgb[3 * i + 0] = col[0];

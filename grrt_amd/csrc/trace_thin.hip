@@ -126,7 +126,10 @@ __device__ __forceinline__ void finish_ray(const TraceParams& P, long long oid, 
         R.col[1] += b;
         R.col[2] += b;
     } else if (term == 1 && P.bg_tex) {
-        texture_sample(P, R.y.v[1], R.y.v[2], R.col);
+        const TexRGB t = texture_sample(P, R.y.v[1], R.y.v[2]);
+        R.col[0] += t.c[0];
+        R.col[1] += t.c[1];
+        R.col[2] += t.c[2];
     }
     const int cls = (R.n_cross > 0) ? GRRT_CLASS_DISK
                                     : (term == 0 ? GRRT_CLASS_CAPTURED

@@ -701,9 +701,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(kWaves))) vo
                 // stars: the thin tracer's catalogue search (src/renderer.cpp:56-61)
                 if (term == 1 && P.stars) mine += star_sample(P, s.y.v[1], s.y.v[2]);
                 if (term == 1 && P.bg_tex) {      // texture background: this lane's channel
-                    double t[3] = {0.0, 0.0, 0.0};
-                    texture_sample(P, s.y.v[1], s.y.v[2], t);
-                    mine += t[c];
+                    const TexRGB t = texture_sample(P, s.y.v[1], s.y.v[2]);
+                    mine += c == 0 ? t.c[0] : (c == 1 ? t.c[1] : t.c[2]);
                 }
                 P.sample_rgb[3 * id + c] = mine;
             }
