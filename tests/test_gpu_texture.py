@@ -125,3 +125,62 @@ def test_unreadable_texture_fails_create(grrt, tmp_path):
     bad.write_bytes(b"#?RADIANCE\n\n+Y 2 +X 2\n")
     with pytest.raises(RuntimeError, match="resolution line"):
         grrt.Context(grrt.make_params(width=16, height=16, texture=str(bad)))
+
+
+def _const_map(tmp_path, colour):
+    path = str(tmp_path / "const.pfm")
+    write_pfm(path, np.broadcast_to(np.float32(colour), (4, 8, 3)))
+    return path
+
+
+def test_batch_views_take_the_texture(grrt, tmp_path):
+    """grrt_ext_render_batch: every escaped pixel of every view carries the
+    constant map's colour, captured pixels equal the batch without it, and
+    disk pixels gain the colour exactly where their ray escaped afterwards."""
+    print("case: batch of 3 views with a constant texture", flush=True)
+    colour = [0.25, 0.5, 1.0]
+    views = [(0.998, 50.0, math.radians(80.0), 0.0, math.radians(90.0), 96, 64),
+             (0.0, 30.0, math.radians(60.0), 0.0, math.radians(70.0), 40, 72),
+             (-0.5, 80.0, math.radians(88.0), 0.3, math.radians(40.0), 128, 48)]
+    kw = dict(width=64, height=64, metric=1, spin=0.5, disk=True, disk_outer=20.0)
+    out = {}
+    for tex in (_const_map(tmp_path, colour), None):
+        with grrt.Context(grrt.make_params(texture=tex, **kw)) as ctx:
+            ctx.set_sampling(grrt.SAMPLING_CENTER)
+            frames = ctx.render_batch(views)
+            out[tex is None] = (np.concatenate([f[..., :3].reshape(-1, 3) for f in frames]), ctx.classes())
+    (rgb_t, cls_t), (rgb_b, cls_b) = out[False], out[True]
+    assert np.array_equal(cls_t, cls_b)
+    esc = cls_t == grrt.CLASS_ESCAPED
+    print(f"escaped pixels {esc.sum()} of {esc.size}", flush=True)
+    assert esc.sum() > 1000
+    assert np.array_equal(rgb_t[esc], np.broadcast_to(np.float32(colour), (esc.sum(), 3)))
+    cap = cls_t == grrt.CLASS_CAPTURED
+    assert np.array_equal(rgb_t[cap], rgb_b[cap])
+    # disk pixels: the disk's light, plus the map's colour where the ray
+    # escaped after crossing the disk (to the float rounding of the sum)
+    disk = cls_t == grrt.CLASS_DISK
+    d = rgb_t[disk].astype(np.float64) - rgb_b[disk]
+    tol = 4e-7 * np.abs(rgb_t[disk]).astype(np.float64) + 1e-6
+    plain = np.all(np.abs(d) <= tol, 1)
+    lit = np.all(np.abs(d - colour) <= tol, 1)
+    print(f"disk pixels {disk.sum()}: unchanged {plain.sum()}, plus the map {lit.sum()}", flush=True)
+    assert np.all(plain | lit) and lit.any()
+
+
+def test_samples_per_pixel_average_the_texture(grrt, tmp_path):
+    """Four seeded samples per pixel: every escaped sample carries the map's
+    colour, and a pixel whose four samples all escaped is exactly that colour."""
+    print("case: 64^2 x 4 spp seeded, constant texture", flush=True)
+    colour = [0.125, 0.75, 2.0]
+    p = grrt.make_params(width=64, height=64, metric=1, spin=0.9, disk=True, texture=_const_map(tmp_path, colour))
+    with grrt.Context(p) as ctx:
+        ctx.set_sampling(grrt.SAMPLING_SEEDED, 4, 11)
+        fb = ctx.render()
+        cls = ctx.classes()
+        rgb = ctx.samples()
+    esc = cls == grrt.CLASS_ESCAPED
+    assert np.array_equal(rgb[esc].astype(np.float32), np.broadcast_to(np.float32(colour), (esc.sum(), 3)))
+    all_esc = esc.reshape(64 * 64, 4).all(1)
+    assert all_esc.sum() > 500
+    assert np.array_equal(fb[..., :3].reshape(-1, 3)[all_esc], np.broadcast_to(np.float32(colour), (all_esc.sum(), 3)))
