@@ -9,12 +9,13 @@ charge-neutral root; the mu and electron-scattering tables, against the same
 map; the absorption table, against the point opacities; the Rosseland table,
 against adaptive quadrature where its band holds the dB/dT peak; the
 hydrogen bound-free edge, against Kramers' threshold cross-section; the
-vertical temperature profile, against its closed form (the vertical
-density profile and the radial flux are pinned in test_vol_structure_pins.py).
-The remaining frequency-dependent opacity terms (free-free, H-), the
-Rosseland table at other temperatures, the radial scale height and midplane
-density have no closed form and the reference cannot be built here: beyond
-the checks below they are "parity unpinned"."""
+hydrogenic free-free absorption, against thermal bremsstrahlung (Rybicki &
+Lightman) over the Saha populations; the vertical temperature profile,
+against its closed form (the vertical density profile and the radial flux
+are pinned in test_vol_structure_pins.py).  The H- terms (tabulated cross
+sections), the Rosseland table at other temperatures, the radial scale
+height and midplane density have no closed form and the reference cannot be
+built here: beyond the checks below they are "parity unpinned"."""
 import math
 
 import numpy as np
@@ -409,3 +410,43 @@ def test_hydrogen_edge_is_the_kramers_threshold_cross_section(g):
         stim = -math.expm1(-K.h * 1e7 * nu_h / (K.k * 1e7 * T))
         jump = (above - below) / (g.saha(rho, T)["n_HI"] * stim)
         assert jump == pytest.approx(7.91e-18, rel=1e-5), (rho, T)
+
+
+def _bremsstrahlung_coefficient():
+    """Rybicki & Lightman (1979) eq. 5.18b: the thermal free-free absorption
+    coefficient 4 e^6 / (3 m h c) (2 pi / (3 k m))^(1/2), in CGS from the
+    SI constants (3.692e8)."""
+    from scipy import constants as K
+    e = K.e * K.c * 10.0                          # statcoulomb
+    m, h, c, k = K.m_e * 1e3, K.h * 1e7, K.c * 1e2, K.k * 1e7
+    return 4.0 * e ** 6 / (3.0 * m * h * c) * math.sqrt(2.0 * math.pi / (3.0 * k * m))
+
+
+def test_free_free_is_thermal_bremsstrahlung(g):
+    """alpha_ff = C T^-1/2 n_e sum_i Z_i^2 n_i nu^-3 (1 - exp(-h nu / k T)) g_ff
+    with C from the fundamental constants (Rybicki & Lightman eq. 5.18b) and
+    the Gaunt factor of the hnu << kT regime, (sqrt 3 / pi) ln(4 k T /
+    (zeta h nu)) with zeta = e^gamma (their section 5.2), which the
+    reference floors at 1.  n_e and sum Z^2 n_i come from the Saha map
+    written in this file.  The grid spans Gaunt factors from 1 to ~13 and
+    h nu / k T from 1e-9 to ~10; dropping the stimulated-emission factor,
+    the 4 in the logarithm, or taking nu^-2 or T^+1/2 each misses by more
+    than 5% somewhere on it."""
+    from scipy import constants as K
+    C = _bremsstrahlung_coefficient()
+    assert C == pytest.approx(3.692e8, rel=1e-3)
+    h, k = K.h * 1e7, K.k * 1e7
+    zeta = math.exp(np.euler_gamma)
+    worst, gmax = 0.0, 1.0
+    for rho in (1e-12, 1e-8):
+        for T in (8000.0, 2e4, 1e6, 1e8):
+            st, _ = _saha_passes(rho, T)
+            for nu in (1e9, 1e12, 1e14, 1e15, 3e15, 1e17):
+                gff = max(math.sqrt(3.0) / math.pi * math.log(4.0 * k * T / (zeta * h * nu)), 1.0)
+                gmax = max(gmax, gff)
+                want = (C * st["n_e"] * st["n_ion_eff"] / nu ** 3 * gff / math.sqrt(T)
+                        * -math.expm1(-h * nu / (k * T)))
+                got = g.opacity(nu, rho, T)["alpha_ff"]
+                worst = max(worst, abs(got / want - 1.0))
+    assert gmax > 10.0
+    assert worst < 1e-3, worst
