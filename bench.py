@@ -418,7 +418,7 @@ def main() -> int:
     ap.add_argument("--tune", default="", help="block,blocks_per_cu,refill_min,chunk[,tile]")
     ap.add_argument("--no-surrogate", action="store_true")
     ap.add_argument("--sg-steps", type=int, default=60000, help="surrogate training steps")
-    ap.add_argument("--big-frames", type=int, default=2, help="2048^2 frames timed per form (0: skip)")
+    ap.add_argument("--big-frames", type=int, default=4, help="2048^2 frames timed per form (0: skip)")
     ap.add_argument("--batch-reps", type=int, default=2, help="replays of the 64-view batch (0: skip)")
     ap.add_argument("--cube-reps", type=int, default=3, help="volumetric spectral cubes timed per form (0: skip)")
     args = ap.parse_args()
