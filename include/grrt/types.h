@@ -19,6 +19,9 @@ typedef enum {
     GRRT_BACKEND_CUDA = 1
 } GRRTBackend;
 
+/* GRRT_BG_TEXTURE with background_texture_path naming an equirectangular
+ * Radiance .hdr or PFM image samples it at each escaped ray's direction;
+ * with no path it is black, as the reference renders the mode. */
 typedef enum {
     GRRT_BG_BLACK = 0,
     GRRT_BG_STARS = 1,
