@@ -12,10 +12,11 @@ hydrogen bound-free edge, against Kramers' threshold cross-section; the
 hydrogenic free-free absorption, against thermal bremsstrahlung (Rybicki &
 Lightman) over the Saha populations; the vertical temperature profile,
 against its closed form (the vertical density profile and the radial flux
-are pinned in test_vol_structure_pins.py).  The H- terms (tabulated cross
-sections), the Rosseland table at other temperatures, the radial scale
-height and midplane density have no closed form and the reference cannot be
-built here: beyond the checks below they are "parity unpinned"."""
+are pinned in test_vol_structure_pins.py); the H- terms' scaling with
+n_HI n_e and their photodetachment threshold.  The H- cross-section tables
+(data), the Rosseland table at other temperatures, the radial scale height
+and midplane density have no closed form and the reference cannot be built
+here: beyond the checks below they are "parity unpinned"."""
 import math
 
 import numpy as np
@@ -450,3 +451,38 @@ def test_free_free_is_thermal_bremsstrahlung(g):
                 worst = max(worst, abs(got / want - 1.0))
     assert gmax > 10.0
     assert worst < 1e-3, worst
+
+
+def test_hminus_scales_with_n_HI_n_e_and_ends_at_photodetachment(g):
+    """Both H- terms are proportional to n_HI n_e: the free-free one
+    directly, the bound-free one through the Saha population n_H- = n_HI n_e
+    lambda_th^3 / 4 exp(chi / k T) (pinned above).  So alpha_H- / (n_HI n_e)
+    depends on (nu, T) only, whatever the density.  Its bound-free part ends
+    at the photodetachment threshold lambda_0 = h c / chi(H-), 0.754 eV:
+    beyond the threshold the ratio is the free-free cross-section alone,
+    which the tables hold constant past 1000 nm, and just inside it the
+    bound-free part is present.  The cross-section tables themselves are
+    data (Wishart 1979, Bell & Berrington 1987)."""
+    from scipy import constants as K
+    lam0_nm = K.h * K.c / (CHI_HMINUS_EV * K.e) * 1e9
+    assert lam0_nm == pytest.approx(1644.3, abs=0.2)
+    c_cm = K.c * 1e2
+    for T in (3000.0, 5000.0, 7000.0):
+        for lam_nm in (450.0, 800.0, 1500.0, 2000.0):
+            nu = c_cm / (lam_nm * 1e-7)
+            ratios = []
+            for rho in (1e-12, 1e-10, 1e-8):
+                st = g.saha(rho, T)
+                ratios.append(g.opacity(nu, rho, T)["alpha_hminus"] / (st["n_HI"] * st["n_e"]))
+            assert max(ratios) / min(ratios) - 1.0 < 1e-9, (T, lam_nm, ratios)
+        def ratio(lam_nm, rho=1e-10):
+            st = g.saha(rho, T)
+            return g.opacity(c_cm / (lam_nm * 1e-7), rho, T)["alpha_hminus"] / (st["n_HI"] * st["n_e"])
+        beyond = [ratio(lam0_nm * f) for f in (1.001, 1.2, 2.0, 5.0)]
+        assert max(beyond) / min(beyond) - 1.0 < 1e-12, (T, beyond)   # free-free alone, constant
+        inside = ratio(lam0_nm * 0.95) - beyond[0]
+        assert inside > 0.0                                            # bound-free present
+        # the bound-free part falls to zero at the threshold (the table's
+        # last node, 1642 nm, is lambda_0 to 0.15%)
+        near = ratio(1641.5) - beyond[0]
+        assert 0.0 < near < 0.01 * inside, (T, near, inside)
