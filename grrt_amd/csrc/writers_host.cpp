@@ -316,12 +316,16 @@ namespace {
     throw std::runtime_error("background texture " + path + ": " + why);
 }
 
-// One header line (without the newline); false at end of file.
+// One header line (without the newline, or a CR LF pair); false at end of
+// file.
 bool read_line(std::FILE* f, std::string& line) {
     line.clear();
     int c;
     while ((c = std::fgetc(f)) != EOF) {
-        if (c == '\n') return true;
+        if (c == '\n') {
+            if (!line.empty() && line.back() == '\r') line.pop_back();
+            return true;
+        }
         line.push_back(static_cast<char>(c));
         if (line.size() > 4096) return false;
     }

@@ -209,6 +209,14 @@ def test_texture_reader_pfm(g, tmp_path, little):
     assert np.array_equal(g.read_texture(path), np.repeat(grey[..., None], 3, -1))
 
 
+def test_texture_reader_accepts_crlf_headers(g, tmp_path):
+    img = np.arange(2 * 3 * 3, dtype=np.float32).reshape(2, 3, 3)
+    path = str(tmp_path / "crlf.pfm")
+    with open(path, "wb") as f:
+        f.write(b"PF\r\n3 2\r\n-1.0\r\n" + np.ascontiguousarray(img[::-1]).astype("<f4").tobytes())
+    assert np.array_equal(g.read_texture(path), img)
+
+
 def test_texture_reader_clears_negative_and_non_finite(g, tmp_path):
     from parity_util import write_pfm
     img = np.full((2, 3, 3), 0.5, np.float32)
