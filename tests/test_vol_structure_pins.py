@@ -20,7 +20,10 @@ the CPU:
   level along these isothermal columns);
 * the radial flux integral (the reference's integrand over [r_isco, r]) at
   every radial node, against adaptive quadrature with analytic derivatives
-  of the circular-orbit E(r), L(r), Omega(r).
+  of the circular-orbit E(r), L(r), Omega(r);
+* the radial scale height and midplane density, against the thin alpha-disk
+  relations H = c_s / Omega_z and local dissipation balance
+  (nu Sigma (r dOmega/dr)^2 proportional to T_eff^4).
 No GPU is needed: the tables come from the library's host build."""
 import math
 
@@ -303,3 +306,41 @@ def test_radial_flux_integral_against_quadrature(disk):
     # the trapezoid's first intervals carry a large relative error where
     # the flux starts from zero at the ISCO; beyond r_isco + M it is small
     assert err[far].max() < 3e-4 and np.median(err) < 1e-4
+
+
+def test_scale_height_and_midplane_density_are_the_alpha_disk_relations(disk):
+    """The radial structure (src/volumetric_disk.cpp:348-483) against the
+    thin alpha-disk relations it implements:
+      * H = c_s / Omega_z, the isothermal sound speed c_s^2 = k T_eff /
+        (mu m_p) over the vertical epicyclic frequency of circular Kerr
+        orbits, Omega_z^2 = Omega^2 (1 - 4 a r^-3/2 + 3 a^2 r^-2) with
+        Omega = 1 / (r^3/2 + a) (Okazaki, Kato & Fukue 1987; M = 1);
+      * local dissipation balance: the viscous heating nu Sigma (r dOmega/dr)^2,
+        with nu = alpha c_s H and Sigma = sqrt(2 pi) H rho_mid, equals the
+        emitted flux F = sigma T_eff^4 up to one constant (the disk's F_max
+        over t_peak^4), so rho_mid H^2 c_s (r dOmega/dr)^2 / T_eff^4 is the
+        same at every radius.
+    Over radii outside the ISCO where H is not clamped to [0.001, 5] M.  The
+    disk is fully ionised there (T_eff > 1e6 K), so mu is the Saha value at
+    any density.  Taking the orbital for the vertical frequency, mu = 1, or
+    Sigma = 2 H rho_mid with rho_mid proportional to 1/H each breaks one of
+    the two by far more than the bars."""
+    import grrt_amd as g
+    from scipy import constants as K
+    D = disk
+    M, a, N = D["M"], D["a"], int(D["n_r"])
+    assert M == 1.0
+    r = D["r_min"] + (D["r_outer"] - D["r_min"]) * np.arange(N) / (N - 1)
+    T, H, rho = D["T_eff"], D["H"], D["rho_mid"]
+    sel = (r > D["r_isco"] + 0.05) & (T > 2e5) & (rho > 0.0) & (H > 0.001 * 1.001) & (H < 5.0 * 0.999)
+    assert sel.sum() > 300
+    r, T, H, rho = r[sel], T[sel], H[sel], rho[sel]
+    om = 1.0 / (r ** 1.5 + a)
+    omz = om * np.sqrt(1.0 - 4.0 * a / r ** 1.5 + 3.0 * a * a / (r * r))
+    mu = np.array([g.saha(1e-10, t)["mu"] for t in T])
+    assert np.ptp(mu) / mu.mean() < 1e-6                       # fully ionised
+    cs = np.sqrt(K.k * 1e7 * T / (mu * K.m_p * 1e3)) / (K.c * 1e2)
+    assert np.abs(H / (cs / omz) - 1.0).max() < 1e-5
+    dom = -1.5 * np.sqrt(r) / (r ** 1.5 + a) ** 2               # d Omega / dr
+    q = rho * H * H * cs * (r * dom) ** 2 / T ** 4
+    assert q.max() / q.min() - 1.0 < 1e-9, (q.min(), q.max())

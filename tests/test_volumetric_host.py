@@ -13,10 +13,11 @@ hydrogenic free-free absorption, against thermal bremsstrahlung (Rybicki &
 Lightman) over the Saha populations; the vertical temperature profile,
 against its closed form (the vertical density profile and the radial flux
 are pinned in test_vol_structure_pins.py); the H- terms' scaling with
-n_HI n_e and their photodetachment threshold.  The H- cross-section tables
-(data), the Rosseland table at other temperatures, the radial scale height
-and midplane density have no closed form and the reference cannot be built
-here: beyond the checks below they are "parity unpinned"."""
+n_HI n_e and their photodetachment threshold (the radial scale height and
+midplane density are pinned in test_vol_structure_pins.py).  The H-
+cross-section tables (data) and the Rosseland table at other temperatures
+have no closed form and the reference cannot be built here: beyond the
+checks below they are "parity unpinned"."""
 import math
 
 import numpy as np
