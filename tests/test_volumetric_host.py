@@ -8,7 +8,7 @@ independently written Saha map and, where the passes converge, against the
 charge-neutral root; the mu and electron-scattering tables, against the same
 map; the absorption table, against the point opacities; the Rosseland table,
 against adaptive quadrature where its band holds the dB/dT peak and, above
-~4e6 K, against the same quadrature times the 50-node rule's closed-form
+~5e5 K, against the same quadrature times the 50-node rule's closed-form
 endpoint factor; the
 hydrogen bound-free edge, against Kramers' threshold cross-section; the
 hydrogenic free-free absorption, against thermal bremsstrahlung (Rybicki &
@@ -17,7 +17,7 @@ against its closed form (the vertical density profile and the radial flux
 are pinned in test_vol_structure_pins.py); the H- terms' scaling with
 n_HI n_e and their photodetachment threshold (the radial scale height and
 midplane density are pinned in test_vol_structure_pins.py).  The H-
-cross-section tables (data) and the Rosseland table between ~1e4 and ~4e6 K
+cross-section tables (data) and the Rosseland table between ~1e4 and ~5e5 K
 have no closed form and the reference cannot be built here: beyond the
 checks below they are "parity unpinned"."""
 import math
@@ -383,8 +383,8 @@ def test_rosseland_table_is_the_rosseland_mean_where_the_band_holds_it(g, disk):
     times larger, and a B_nu-weighted harmonic mean differs by 9-42%.  At
     other temperatures the 50-point rule differs from the continuum integral
     by up to ~20% (an absorption edge between its frequencies, or the band
-    cutting the dB/dT peak above ~2e4 K).  Above ~4e6 K the next test pins
-    the rule's offset in closed form; between ~1e4 and ~4e6 K the table is
+    cutting the dB/dT peak above ~2e4 K).  Above ~5e5 K the next test pins
+    the rule's offset in closed form; between ~1e4 and ~5e5 K the table is
     the reference's rule and is checked only through the opacity lookups
     downstream."""
     from scipy.integrate import quad
@@ -494,25 +494,27 @@ def test_hminus_scales_with_n_HI_n_e_and_ends_at_photodetachment(g):
 
 
 def test_rosseland_table_at_high_temperature_is_the_band_integral_with_its_endpoint_factor(g, disk):
-    """Above ~4e6 K the gas is fully ionised and the band 1e13..1e16 Hz is in
-    the Rayleigh-Jeans tail: both Rosseland integrands, per log10 nu, grow
+    """Above ~5e5 K the band 1e13..1e16 Hz lies below the dB/dT peak, in the
+    Rayleigh-Jeans tail or close to it, above every absorption edge's effect: both Rosseland integrands, per log10 nu, grow
     almost exponentially towards the band's top, f ~ e^(k x) (k = ln10 times
     the local power of nu: ~5.2 for (1/kappa) dB/dT nu, 3 for dB/dT nu).  A
     sum over 50 log-uniform nodes spaced h = 3/49 with the end nodes at full
     weight then exceeds the integral by the factor k h / (1 - e^(-k h)),
     different for the two integrals, so the table's kappa_R is the
     continuum value (adaptive quadrature of the point opacities over the
-    band) times factor(denominator) / factor(numerator), ~0.86-0.87.  The
-    slopes are measured at the band's top.  Agreement within 0.3%; a
-    trapezoid rule would give ~1.0 and a Planck mean hundreds of times more."""
+    band) times factor(denominator) / factor(numerator), ~0.85-0.87.  The
+    slopes are measured at the band's top.  Agreement within 0.5% (0.14%
+    above 4e6 K); a trapezoid rule would give ~1.0 and a Planck mean
+    hundreds of times more.  Below ~5e5 K the model fails (1.5% at 2e5 K,
+    13% at 2.5e4 K): edges and the dB/dT peak enter the band."""
     from scipy.integrate import quad
     n, h = 100, 3.0 / 49.0
     fac = lambda k: k * h / -math.expm1(-k * h)
     checked = 0
     for j in (10, 50, 90):
-        for k in (75, 85, 99):
+        for k in (50, 70, 99):
             rho, T = _grid_point(j, k)
-            assert T > 4e6
+            assert T > 5e5
             kap = lambda lnu: max(g.opacity(10.0 ** lnu, rho, T)["kappa_abs"], 1e-300)
             w = lambda lnu: _dplanck_dT(10.0 ** lnu, T) * 10.0 ** lnu
             fn = lambda lnu: w(lnu) / kap(lnu)
@@ -522,6 +524,6 @@ def test_rosseland_table_at_high_temperature_is_the_band_integral_with_its_endpo
             num = quad(fn, 13.0, 16.0, limit=400)[0]
             den = quad(w, 13.0, 16.0, limit=400)[0]
             want = den / num * fac(kd) / fac(kn)
-            assert disk["kappa_ross"][j * n + k] == pytest.approx(want, rel=3e-3), (rho, T)
+            assert disk["kappa_ross"][j * n + k] == pytest.approx(want, rel=5e-3), (rho, T)
             checked += 1
     assert checked == 9
