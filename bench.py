@@ -224,8 +224,10 @@ def surrogate_section(grrt, ctx, plain_value, steps, warmup, sg_steps, big_frame
             med_big, within_big = pixel_flux_error(f_big, plain_big, disk_pixels(plain_big_cls, sg_big_cls))
             cbig.render_surrogate(SG_ENTROPY, SG_JUDGE, SG_LNG_TOL, device=True)
             t0 = time.perf_counter()
+            big_stats = []
             for _ in range(big_frames):
                 _, st_big = cbig.render_surrogate(SG_ENTROPY, SG_JUDGE, SG_LNG_TOL, device=True)
+                big_stats.append(st_big)
             sg_ms = (time.perf_counter() - t0) / big_frames * 1e3
         rays_big = 2048 * 2048 * SPP
         out["frame_2048sq_4spp"] = {
@@ -234,7 +236,10 @@ def surrogate_section(grrt, ctx, plain_value, steps, warmup, sg_steps, big_frame
             "class_agreement_vs_plain": round(float(np.mean(sg_big_cls == plain_big_cls)), 6),
             "median_disk_pixel_flux_rel_err_vs_plain": round(med_big, 5),
             "fallback_frac": round(st_big["fallback_rays"] / st_big["rays"], 5),
-            "judge_median_rel_err": round(st_big["judge_median_rel_err"], 5)}
+            "judge_median_rel_err": round(st_big["judge_median_rel_err"], 5),
+            "gpu_ms_per_frame": round(sum(x["gpu_ms"] for x in big_stats) / big_frames, 3),
+            "inference_span_ms": round(sum(x["infer_ms"] for x in big_stats) / big_frames, 3),
+            "trace_after_inference_ms": round(sum(x["trace_ms"] for x in big_stats) / big_frames, 3)}
     return out
 
 

@@ -669,6 +669,10 @@ int grrt_ext_render_surrogate(GRRTContext* ctx, float* out, int device_out, doub
             IA.tile_counter = S.counts.ptr + 4 * k + 3;
             set_view_features(IA, P.view);
             check(launch_mlp_infer(IA, ctx->num_cus, st), "surrogate pass");
+            // The end of the pass is recorded before any work goes to the
+            // second stream: where the two streams share one hardware queue,
+            // an event recorded later would land after that work.
+            if (k == nb - 1) check(hipEventRecord(ctx->ev[1], st), "event");
             check(hipEventRecord(ctx->band_ev[k], st), "event");
             check(hipStreamWaitEvent(s2, ctx->band_ev[k], 0), "stream wait");
             check(launch_trace_thin(ctx->params_dev.ptr + k, 2, blocks, ctx->tune_block, s2, ctx->integrator),
@@ -678,7 +682,6 @@ int grrt_ext_render_surrogate(GRRTContext* ctx, float* out, int device_out, doub
                                S.judge_err.ptr + bands[k].joff, S.counts.ptr + 4 * k + 2, s2),
                   "judge");
         }
-        check(hipEventRecord(ctx->ev[1], st), "event");
         check(launch_finalize(ctx->sample_rgb.ptr, ctx->sample_cls.ptr, ctx->frame.ptr, npix, nS, s2), "finalize");
         check(hipEventRecord(ctx->ev[2], s2), "event");
         check(hipEventSynchronize(ctx->ev[2]), "surrogate frame");
